@@ -1,0 +1,100 @@
+/*
+ * tcstep.h -- handle-based C-ABI of the MI355X RK3 step library (libtcstep.so).
+ *
+ * The reference keeps one solver per process in module statics that are initialised on the
+ * first call (src/fluid/compute_potential.c:505-514, src/fluid/predict/ux.c:455-474,
+ * src/decide_dt.c:21-23).  This library keeps the same state in an explicit handle, so a
+ * process can hold several independent solvers (the ensemble driver does).  The
+ * reference-shaped entry points (integrate.h, fluid_solver.h, decide_dt.h) are thin
+ * wrappers over a default handle.
+ *
+ * Conventions (the reference's, src/*.c): every function returns 0 on success and non-zero
+ * on failure; tc_last_error() then returns a message.  Fields cross this boundary in the
+ * reference's NPY layout (src/array.c:82-222): ux is (nz, ny, nx+1) with the x walls at
+ * i = 1 and nx+1, uy/uz/p/psi/t are (nz, ny, nx+2) with the walls at i = 0 and nx+1, no
+ * y/z halos, C order, float64.  Device buffers stay resident between calls; nothing moves
+ * between host and device except through tc_set_field / tc_get_field and the scalar dt.
+ */
+#ifndef TCSTEP_H
+#define TCSTEP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_FP64 0
+
+typedef struct {
+  int nx, ny, nz;           /* glsizes (domain.c:40) */
+  int is_curved;            /* domain.c:37 */
+  double lengths[3];        /* domain.c:43 */
+  double Re, Pr;            /* fluid/init.c:118-119 */
+  int implicit[3];          /* param/implicit.c:4-6, default {1, 0, 0} */
+  int scalar;               /* 1: advance T (reference), 0: T is left untouched */
+  double uy_xm, uy_xp;      /* param/boundary-condition.c:3-4, default 1, 0 */
+  double t_xm, t_xp;        /* param/boundary-condition.c:6-7, default +0.5, -0.5 */
+  double coef_dt_adv;       /* decide_dt.c:128 */
+  double coef_dt_dif;       /* decide_dt.c:129 */
+  double dt_fixed;          /* > 0: decide_dt returns this value (no reference counterpart) */
+  int precision;            /* TC_FP64 */
+  int device;               /* HIP device ordinal */
+} tc_params_t;
+
+typedef struct tc_solver_s tc_solver_t;
+
+/* defaults of the reference (src/param/*.c), Re = Pr = 1, coefficients 1 */
+void tc_params_default(tc_params_t * p);
+
+/* xf: nx+1 face positions XF(1..nx+1), xc: nx+2 centre positions XC(0..nx+1)
+ * (domain.c:46-53).  Fields start at zero with boundary values applied. */
+int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_solver_t ** out);
+int tc_destroy(tc_solver_t * s);
+const char * tc_last_error(void);
+
+/* name: "ux" "uy" "uz" "p" "psi" "t"; data in the NPY layout above.  tc_set_field applies the
+ * field's boundary condition and periodic halos (fluid/init.c:112-116). */
+int tc_set_field(tc_solver_t * s, const char * name, const double * data);
+int tc_get_field(tc_solver_t * s, const char * name, double * data);
+
+/* one time step, integrate() (integrate.c:9-35); *dt receives the step size used */
+int tc_step(tc_solver_t * s, double * dt);
+/* nsteps steps queued on the solver's stream with no host synchronisation */
+int tc_step_async(tc_solver_t * s, int nsteps);
+int tc_synchronize(tc_solver_t * s);
+int tc_get_dt(tc_solver_t * s, double * dt);
+
+/* decide_dt (decide_dt.c:125-160) on the current field */
+int tc_decide_dt(tc_solver_t * s, double * dt);
+/* the four stages of one RK substep with a host-given dt (fluid_solver.h:34-60) */
+int tc_predict(tc_solver_t * s, int rkstep, double dt);
+int tc_potential(tc_solver_t * s, int rkstep, double dt);
+int tc_correct(tc_solver_t * s, int rkstep, double dt);
+int tc_pressure(tc_solver_t * s, int rkstep, double dt);
+
+/* diagnostics in FP64 (src/logging/*.c).  Injection and dissipation use the velocity
+ * gradients of the last predict call, as the reference's logging reads the tensors it left
+ * (injection.c:49-51); before any step they use the current field. */
+int tc_divergence(tc_solver_t * s, double * divmax);
+int tc_injection(tc_solver_t * s, double out[2]);
+int tc_dissipation(tc_solver_t * s, double out[2]);
+int tc_total_energy(tc_solver_t * s, double out[4]);
+/* on: every following step records the injection and dissipation of the field at the start
+ * of its last substep (the reference's logging reads the tensors that substep left) */
+int tc_arm_diagnostics(tc_solver_t * s, int on);
+
+/* events around the kernels of one named stage ("predict", "poisson", ...) while timing */
+int tc_profile_enable(tc_solver_t * s, int on);
+int tc_profile_read(tc_solver_t * s, const char * stage, double * total_ms, long long * launches);
+size_t tc_device_bytes(const tc_solver_t * s);
+
+/* seeded initial condition (tests/icgen.py states the same definition in numpy):
+ * tanh-clustered grid (stretch s, s = 0 uniform) and laminar Couette + uniform noise */
+int tc_ic_grid(int nx, double lx, int is_curved, double stretch, double * xf, double * xc);
+int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,143 @@
+/*
+ * tc_ic.c -- seeded initial conditions in the reference's NPY layout.
+ *
+ * The reference's own generator (initial_condition/main.py:52-113) draws unseeded uniform
+ * noise on a clipped-Chebyshev grid; the workloads here need a seeded, reproducible field on
+ * a tanh-clustered grid with a laminar profile, so this tool defines one:
+ *   grid     xf[i] = ri + lx/2 (1 + tanh(s (2i/nx - 1)) / tanh(s)),  xc as in main.py:71-77
+ *   laminar  circular Couette uy = A r + B / r between the wall values (curved) or the
+ *            linear profile (planar); conduction T = log(r/ro)/log(ri/ro) profile
+ *   noise    amp (2u - 1), u = (splitmix64(key) >> 11) 2^-53,
+ *            key = seed * 0x9E3779B97F4A7C15 + (field << 56) + linear NPY index
+ *   modes    mode_amp X(x) Y(y) Z(z), each factor the mean of nmodes seeded sine/cosine
+ *            modes (vanishing at the walls in x), a smooth "turbulent-like" start
+ * Each field is assembled in a host buffer in the NPY layout and uploaded with
+ * tc_set_field, so the same values reach the device as a file written by the tool would.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <hip/hip_runtime_api.h>
+#include "tcstep.h"
+#include "tc_internal.h"
+
+static unsigned long long splitmix64(unsigned long long z){
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+static double uniform01(unsigned long long seed, unsigned field, unsigned long long lin){
+  const unsigned long long key = seed * 0x9E3779B97F4A7C15ULL + ((unsigned long long)field << 56) + lin;
+  return (double)(splitmix64(key) >> 11) * 0x1.0p-53;
+}
+
+int tc_ic_grid(int nx, double lx, int is_curved, double stretch, double * xf, double * xc){
+  if(nx < 2) return tc_fail("nx too small");
+  const double ri = is_curved ? 1. : 0.;
+  for(int i = 0; i <= nx; i++){
+    const double fi = (double)i;
+    if(stretch > 0.) xf[i] = ri + lx * 0.5 * (1. + tanh(stretch * (2. * fi / nx - 1.)) / tanh(stretch));
+    else xf[i] = ri + lx * fi / nx;
+  }
+  xf[0] = ri;
+  xf[nx] = ri + lx;
+  xc[0] = xf[0];
+  for(int i = 1; i <= nx; i++) xc[i] = 0.5 * xf[i - 1] + 0.5 * xf[i];
+  xc[nx + 1] = xf[nx];
+  return 0;
+}
+
+static double laminar_uy(double r, double ri, double ro, double ui, double uo, int curved){
+  if(curved){
+    const double a = (uo * ro - ui * ri) / (ro * ro - ri * ri);
+    const double b = ri * ro * (ui * ro - uo * ri) / (ro * ro - ri * ri);
+    return a * r + b / r;
+  }
+  return uo + (ui - uo) * (ro - r) / (ro - ri);
+}
+
+static double conduction_t(double r, double ri, double ro, double ti, double to, int curved){
+  if(curved) return to + (ti - to) * log(r / ro) / log(ri / ro);
+  return to + (ti - to) * (ro - r) / (ro - ri);
+}
+
+/* mean of nm seeded modes; kind 0: sin(pi m xi) (walls), 1: cos(2 pi m t / n + phase) */
+static void mode_table(unsigned long long seed, unsigned field, unsigned dir, int nm, int kind,
+    int n, const double * pos, double * out){
+  const double two_pi = 6.283185307179586476925286766559;
+  const double pi = 3.14159265358979323846;
+  for(int q = 0; q < n; q++) out[q] = 0.;
+  for(int m = 1; m <= nm; m++){
+    const unsigned long long tag = ((unsigned long long)dir << 40) + ((unsigned long long)m << 8);
+    const double amp = 2. * uniform01(seed, 16 + field, tag) - 1.;
+    const double phase = two_pi * uniform01(seed, 32 + field, tag);
+    for(int q = 0; q < n; q++){
+      const double v = kind == 0 ? sin(pi * m * pos[q]) : cos(two_pi * m * pos[q] + phase);
+      out[q] += amp * v / nm;
+    }
+  }
+}
+
+int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp){
+  const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+  const int curved = s->p.is_curved;
+  const double * xf = s->h_xf;   /* XF(i) at [i] */
+  const double * xc = s->h_xc;
+  const double ri = xf[1], ro = xf[nx + 1];
+  const double lx = ro - ri;
+  const char * names[5] = {"ux", "uy", "uz", "t", "p"};
+  const size_t plane_max = (size_t)ny * (nx + 2);
+  double * full = NULL;
+  const size_t nfull = (size_t)nz * plane_max;
+  full = malloc(sizeof(double) * nfull);
+  if(!full) return tc_fail("host allocation failed (%zu MB)", sizeof(double) * nfull >> 20);
+  double * X = malloc(sizeof(double) * (nx + 2));
+  double * Y = malloc(sizeof(double) * ny);
+  double * Z = malloc(sizeof(double) * nz);
+  double * pos = malloc(sizeof(double) * ((nx + 2) > ny ? ((nx + 2) > nz ? (nx + 2) : nz) : (ny > nz ? ny : nz)));
+  if(!X || !Y || !Z || !pos){ free(full); free(X); free(Y); free(Z); free(pos); return tc_fail("host allocation failed"); }
+  int rc = 0;
+  for(int f = 0; f < 5 && !rc; f++){
+    const int nxs = f == 0 ? nx + 1 : nx + 2;
+    const int fid = f == 4 ? 4 : f;   /* noise stream: ux 0, uy 1, uz 2, t 3 */
+    /* laminar profile per x index */
+    for(int q = 0; q < nxs; q++){
+      const double r = f == 0 ? xf[q + 1] : xc[q];
+      double v = 0.;
+      if(f == 1) v = laminar_uy(r, ri, ro, s->p.uy_xm, s->p.uy_xp, curved);
+      if(f == 3) v = conduction_t(r, ri, ro, s->p.t_xm, s->p.t_xp, curved);
+      X[q] = v;
+    }
+    const int use_modes = nmodes > 0 && mode_amp != 0. && f < 3;
+    double * MX = NULL;
+    if(use_modes){
+      MX = malloc(sizeof(double) * nxs);
+      for(int q = 0; q < nxs; q++) pos[q] = ((f == 0 ? xf[q + 1] : xc[q]) - ri) / lx;
+      mode_table(seed, f, 0, nmodes, 0, nxs, pos, MX);
+      for(int q = 0; q < ny; q++) pos[q] = (double)q / ny;
+      mode_table(seed, f, 1, nmodes, 1, ny, pos, Y);
+      for(int q = 0; q < nz; q++) pos[q] = (double)q / nz;
+      mode_table(seed, f, 2, nmodes, 1, nz, pos, Z);
+    }
+    const double a = f < 3 ? amp : (f == 3 ? t_amp : 0.);
+    for(int k = 0; k < nz; k++){
+      for(int j = 0; j < ny; j++){
+        double * row = full + ((size_t)k * ny + j) * nxs;
+        const unsigned long long lin0 = ((unsigned long long)k * ny + j) * nxs;
+        for(int q = 0; q < nxs; q++){
+          double v = X[q];
+          if(a != 0.) v = v + a * (2. * uniform01(seed, fid, lin0 + q) - 1.);
+          if(use_modes) v = v + mode_amp * MX[q] * Y[j] * Z[k];
+          row[q] = v;
+        }
+      }
+    }
+    free(MX);
+    rc = tc_set_field(s, names[f], full);
+  }
+  free(full); free(X); free(Y); free(Z); free(pos);
+  return rc;
+}

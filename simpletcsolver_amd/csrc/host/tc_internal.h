@@ -1,0 +1,83 @@
+/*
+ * tc_internal.h -- the solver handle (private to the host library).
+ */
+#ifndef TC_INTERNAL_H
+#define TC_INTERNAL_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+#include "tcstep.h"
+#include "../tc_shared.h"
+
+enum { TC_FUX = 0, TC_FUY = 1, TC_FUZ = 2, TC_FT = 3, TC_FP = 4, TC_FPSI = 5 };
+
+enum { TC_S_DT, TC_S_FACTOR, TC_S_PREDICT, TC_S_XSOLVE, TC_S_YSOLVE, TC_S_ZSOLVE,
+  TC_S_DIVY, TC_S_ZFFT, TC_S_PXSOLVE, TC_S_YIFFT, TC_S_CORRECT, TC_S_DIAG, TC_NSTAGE };
+
+#define TC_MAXALLOC 128
+#define TC_MAXEV 64
+
+/* line-solve factor tables of one field (see tck_linesolve.hip) */
+typedef struct {
+  double * xr; double * xv; double * xl;
+  double * yr; double * yv; double * yq1; double * ymeta;
+  double * zr; double * zv; double * zq1; double * zmeta;
+} tc_ls_t;
+
+typedef struct {
+  hipEvent_t ev[TC_MAXEV][2];
+  int nrec;
+  double total_ms;
+  long long launches;
+} tc_prof_t;
+
+struct tc_solver_s {
+  tc_params_t p;
+  tc_geom_t g;
+  tc_coef_t coef;
+  tc_flags_t flags;
+  hipStream_t stream;
+  int tp;                 /* pitch of per-i tables: nx + 2 */
+  int nky;
+  double hz, nu, kappa;
+  double dif_limit;       /* min of the explicit diffusive dt limits */
+  /* device state */
+  double * u[TC_NFIELD][2];
+  int cur[TC_NFIELD];
+  double * src[TC_NFIELD][2];
+  double * d_p;
+  double * d_psi;
+  double * d_tab;
+  double * d_spec;
+  double * d_pr; double * d_pv; double * d_pl; double * d_pu;
+  double * d_evy; double * d_evz;
+  double * d_twy; double * d_twyn; double * d_twz;
+  double * d_dt;
+  unsigned long long * d_dtmin;
+  double * d_part;
+  double * d_diag;
+  tc_ls_t ls[TC_NFIELD];
+  /* host copies */
+  double * h_tab; double * h_xf; double * h_xc; double * h_pl; double * h_pu;
+  double h_dt;
+  double h_diag_snap[10];
+  double diag_vals[10];
+  int diag_valid, diag_pending, diag_armed;
+  int dt_valid;
+  /* bookkeeping */
+  void * allocs[TC_MAXALLOC];
+  int nalloc;
+  size_t dev_bytes;
+  int prof_on;
+  tc_prof_t prof[TC_NSTAGE];
+};
+
+int tc_fail(const char * fmt, ...);
+int tc_apply_bc(tc_solver_t * s, int field, double * a);
+double * tc_field_ptr(tc_solver_t * s, int f);
+int tc_enqueue_step(tc_solver_t * s);
+
+void tc_prof_begin(tc_solver_t * s, int stage);
+void tc_prof_end(tc_solver_t * s, int stage);
+
+#endif

@@ -1,0 +1,843 @@
+/*
+ * tc_solver.c -- host orchestration of the MI355X RK3 step (C11).
+ *
+ * One tc_solver_t owns the device-resident state of one flow (the reference's fluid_t and
+ * the module statics of its solvers) and queues the kernels of integrate() (integrate.c:9-35)
+ * on one HIP stream:
+ *
+ *   decide_dt      CFL minimum (fused into the previous step's last correction, or its own
+ *                  reduction) + finalise; dt never leaves the device inside a step
+ *   3 x substep    line-solve factor tables (from dt on the device)
+ *                  predict (one fused sweep for ux uy uz T)
+ *                  implicit line solves x -> y -> z, the last one adding the increment
+ *                  Poisson: div + theta rFFT, z FFT, radial solves, z iFFT, theta iFFT -> psi
+ *                  correction + pressure update (+ CFL minimum of the next step)
+ *
+ * Velocity and scalar fields are double-buffered (the predictor reads neighbours of the old
+ * field while writing the new one); the RK source of substep s lives in buffer s % 2 and is
+ * read back at substep s+1, which replaces the reference's pointer swap (predict/main.c:8-25)
+ * by a fixed assignment.
+ */
+#define _GNU_SOURCE
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdarg.h>
+#include <math.h>
+#include <float.h>
+#include <hip/hip_runtime_api.h>
+#include "tcstep.h"
+#include "tc_kernels.h"
+#include "tc_internal.h"
+
+static __thread char g_err[512];
+
+const char * tc_last_error(void){
+  return g_err;
+}
+
+int tc_fail(const char * fmt, ...){
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return 1;
+}
+
+#define HIPOK(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess) \
+  return tc_fail("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); } while(0)
+#define KOK(expr) do { int e_ = (expr); if(e_ != 0) \
+  return tc_fail("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString((hipError_t)e_)); } while(0)
+
+void tc_params_default(tc_params_t * p){
+  memset(p, 0, sizeof(*p));
+  p->is_curved = 1;
+  p->lengths[0] = p->lengths[1] = p->lengths[2] = 1.;
+  p->Re = 1.;
+  p->Pr = 1.;
+  p->implicit[0] = 1;
+  p->scalar = 1;
+  p->uy_xm = 1.;
+  p->uy_xp = 0.;
+  p->t_xm = + 0.5;
+  p->t_xp = - 0.5;
+  p->coef_dt_adv = 1.;
+  p->coef_dt_dif = 1.;
+  p->dt_fixed = 0.;
+  p->precision = TC_FP64;
+  p->device = 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * device memory
+ * ------------------------------------------------------------------------------------- */
+static int dalloc(tc_solver_t * s, double ** p, size_t n){
+  if(s->nalloc >= TC_MAXALLOC) return tc_fail("too many device allocations");
+  const size_t bytes = sizeof(double) * (n ? n : 1);
+  HIPOK(hipMalloc((void **)p, bytes));
+  HIPOK(hipMemsetAsync(*p, 0, bytes, s->stream));
+  s->allocs[s->nalloc++] = *p;
+  s->dev_bytes += bytes;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * metrics and per-i coefficient tables (domain.c:259-357, predict/*.c init_laplacians,
+ * compute_potential.c:173-211), evaluated with the operand order of the reference
+ * ------------------------------------------------------------------------------------- */
+enum {
+  T_HXXF, T_HXXC, T_HYXF, T_HYXC, T_JDXF, T_JDXC,
+  T_AXF, T_IJDXF, T_IHXF, T_IHYF, T_JHXF, T_JHYF, T_JHZF, T_L1F, T_NUJF, T_N2JF, T_D1M, T_D1P,
+  T_AYC, T_AZC, T_IJDXC, T_IHYC, T_JHYC, T_JHZC, T_L1C, T_HDC, T_NUJC, T_N2JC, T_DAM, T_DAP,
+  T_LXU_L, T_LXU_C, T_LXU_U, T_LYU_L, T_LYU_C, T_LYU_U,
+  T_LXC_L, T_LXC_C, T_LXC_U, T_LYC_L, T_LYC_C, T_LYC_U,
+  T_PLX_L, T_PLX_U, T_PLY, T_PLZ,
+  T_NTAB
+};
+
+static int build_tables(tc_solver_t * s, const double * xf_in, const double * xc_in){
+  const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+  const int tp = nx + 2;
+  s->tp = tp;
+  double * xf = calloc((size_t)nx + 2, sizeof(double));   /* XF(i) at [i] */
+  double * xc = calloc((size_t)nx + 2, sizeof(double));
+  double * tab = calloc((size_t)T_NTAB * tp, sizeof(double));
+  if(!xf || !xc || !tab) return tc_fail("host allocation failed");
+  for(int i = 1; i <= nx + 1; i++) xf[i] = xf_in[i - 1];
+  for(int i = 0; i <= nx + 1; i++) xc[i] = xc_in[i];
+  #define TB(t, i) tab[(size_t)(t) * tp + (i)]
+  const double ly = s->p.lengths[1];
+  const double hz = s->p.lengths[2] / nz;
+  s->hz = hz;
+  for(int i = 1; i <= nx + 1; i++) TB(T_HXXF, i) = xc[i] - xc[i - 1];
+  TB(T_HXXC, 0) = 0.5 * xf[2] - 0.5 * xf[1];
+  for(int i = 1; i <= nx; i++) TB(T_HXXC, i) = xf[i + 1] - xf[i];
+  TB(T_HXXC, nx + 1) = 0.5 * xf[nx + 1] - 0.5 * xf[nx];
+  for(int i = 1; i <= nx + 1; i++) TB(T_HYXF, i) = s->p.is_curved ? xf[i] * ly / ny : ly / ny;
+  for(int i = 0; i <= nx + 1; i++) TB(T_HYXC, i) = s->p.is_curved ? xc[i] * ly / ny : ly / ny;
+  for(int i = 1; i <= nx + 1; i++) TB(T_JDXF, i) = TB(T_HXXF, i) * TB(T_HYXF, i) * hz;
+  for(int i = 0; i <= nx + 1; i++) TB(T_JDXC, i) = TB(T_HXXC, i) * TB(T_HYXC, i) * hz;
+  const double nu = 1. / s->p.Re;
+  const double kappa = 1. / s->p.Re / s->p.Pr;
+  s->nu = nu;
+  s->kappa = kappa;
+  #define HXXF(i) TB(T_HXXF, i)
+  #define HXXC(i) TB(T_HXXC, i)
+  #define HYXF(i) TB(T_HYXF, i)
+  #define HYXC(i) TB(T_HYXC, i)
+  #define JDXF(i) TB(T_JDXF, i)
+  #define JDXC(i) TB(T_JDXC, i)
+  for(int i = 1; i <= nx + 1; i++){
+    TB(T_AXF, i) = 0.5 * JDXF(i) / HXXF(i);
+    TB(T_IJDXF, i) = 1. / JDXF(i);
+    TB(T_IHXF, i) = 1. / HXXF(i);
+    TB(T_IHYF, i) = 1. / HYXF(i);
+    TB(T_JHXF, i) = JDXF(i) / HXXF(i);
+    TB(T_JHYF, i) = JDXF(i) / HYXF(i);
+    TB(T_JHZF, i) = JDXF(i) / hz;
+    /* lyx.c:72-79: - 1. / jd_x0 * djdhx, djdhx = - JDXC(i-1)/HXXC(i-1) + JDXC(i)/HXXC(i) */
+    const double jdhx_xm = JDXC(i - 1) / HXXC(i - 1);
+    const double jdhx_xp = JDXC(i) / HXXC(i);
+    const double djdhx = - jdhx_xm + jdhx_xp;
+    TB(T_L1F, i) = - 1. / JDXF(i) * djdhx;
+    TB(T_NUJF, i) = nu / JDXF(i);
+    TB(T_N2JF, i) = 2. * nu / JDXF(i);
+  }
+  for(int i = 2; i <= nx; i++){
+    /* ux.c:206-215 and :358-365 */
+    const double djdhx_xm = - JDXF(i - 1) / HXXF(i - 1) + JDXF(i) / HXXF(i);
+    const double djdhx_xp = - JDXF(i) / HXXF(i) + JDXF(i + 1) / HXXF(i + 1);
+    TB(T_D1M, i) = 0.5 * djdhx_xm;
+    TB(T_D1P, i) = 0.5 * djdhx_xp;
+    /* ux.c:45-51, :59-65 */
+    const double l = 1. / JDXF(i) * JDXC(i - 1) / HXXC(i - 1) / HXXC(i - 1);
+    const double u = 1. / JDXF(i) * JDXC(i) / HXXC(i) / HXXC(i);
+    TB(T_LXU_L, i) = l;
+    TB(T_LXU_C, i) = - l - u;
+    TB(T_LXU_U, i) = u;
+    const double ly_ = 1. / HYXF(i) / HYXF(i);
+    const double uy_ = 1. / HYXF(i) / HYXF(i);
+    TB(T_LYU_L, i) = ly_;
+    TB(T_LYU_C, i) = - ly_ - uy_;
+    TB(T_LYU_U, i) = uy_;
+  }
+  for(int i = 0; i <= nx + 1; i++){
+    TB(T_AYC, i) = 0.5 * JDXC(i) / HYXC(i);
+    TB(T_AZC, i) = 0.5 * JDXC(i) / hz;
+    TB(T_IJDXC, i) = 1. / JDXC(i);
+    TB(T_IHYC, i) = 1. / HYXC(i);
+    TB(T_JHYC, i) = JDXC(i) / HYXC(i);
+    TB(T_JHZC, i) = JDXC(i) / hz;
+    TB(T_NUJC, i) = nu / JDXC(i);
+    TB(T_N2JC, i) = 2. * nu / JDXC(i);
+  }
+  for(int i = 1; i <= nx; i++){
+    /* lyy.c:72-79 and uy.c:205-208 */
+    const double jdhx_xm = JDXF(i) / HXXF(i);
+    const double jdhx_xp = JDXF(i + 1) / HXXF(i + 1);
+    const double djdhx = - jdhx_xm + jdhx_xp;
+    TB(T_L1C, i) = 1. / JDXC(i) * djdhx;
+    const double djdhx2 = - JDXF(i) / HXXF(i) + JDXF(i + 1) / HXXF(i + 1);
+    TB(T_HDC, i) = 0.5 * djdhx2;
+    /* uy.c:384-385 */
+    const double djdhx_xm = - JDXC(i - 1) / HXXC(i - 1) + JDXC(i) / HXXC(i);
+    const double djdhx_xp = - JDXC(i) / HXXC(i) + JDXC(i + 1) / HXXC(i + 1);
+    TB(T_DAM, i) = 0.5 * djdhx_xm;
+    TB(T_DAP, i) = 0.5 * djdhx_xp;
+    /* uy.c:46-52, :58-64 */
+    const double l = 1. / JDXC(i) * JDXF(i) / HXXF(i) / HXXF(i);
+    const double u = 1. / JDXC(i) * JDXF(i + 1) / HXXF(i + 1) / HXXF(i + 1);
+    TB(T_LXC_L, i) = l;
+    TB(T_LXC_C, i) = - l - u;
+    TB(T_LXC_U, i) = u;
+    const double ly_ = 1. / HYXC(i) / HYXC(i);
+    const double uy_ = 1. / HYXC(i) / HYXC(i);
+    TB(T_LYC_L, i) = ly_;
+    TB(T_LYC_C, i) = - ly_ - uy_;
+    TB(T_LYC_U, i) = uy_;
+    /* update_pressure.c:59-60, :91-92, :122-123 */
+    TB(T_PLX_L, i) = JDXF(i) / HXXF(i) / HXXF(i);
+    TB(T_PLX_U, i) = JDXF(i + 1) / HXXF(i + 1) / HXXF(i + 1);
+    TB(T_PLY, i) = JDXC(i) / HYXC(i) / HYXC(i);
+    TB(T_PLZ, i) = JDXC(i) / hz / hz;
+  }
+  /* decide_dt_dif (decide_dt.c:89-115) for the explicit directions */
+  {
+    const double diffusivity = fmax(nu, kappa);
+    double grid[3];
+    grid[0] = DBL_MAX;
+    for(int i = 0; i <= nx + 1; i++) grid[0] = fmin(grid[0], HXXC(i));
+    grid[1] = HYXC(1);
+    grid[2] = hz;
+    double lim = INFINITY;
+    for(int d = 0; d < 3; d++){
+      const double dif = s->p.coef_dt_dif / diffusivity * 0.5 / 3 * pow(grid[d], 2.);
+      if(!s->p.implicit[d]) lim = fmin(lim, dif);
+    }
+    s->dif_limit = lim;
+  }
+  /* Poisson tables (compute_potential.c:205-209, :311-349) */
+  s->h_pl = calloc((size_t)nx, sizeof(double));
+  s->h_pu = calloc((size_t)nx, sizeof(double));
+  for(int i = 1; i <= nx; i++){
+    s->h_pl[i - 1] = 1. / JDXC(i) * JDXF(i) / HXXF(i) / HXXF(i);
+    s->h_pu[i - 1] = 1. / JDXC(i) * JDXF(i + 1) / HXXF(i + 1) / HXXF(i + 1);
+  }
+  #undef HXXF
+  #undef HXXC
+  #undef HYXF
+  #undef HYXC
+  #undef JDXF
+  #undef JDXC
+  /* device copy */
+  if(dalloc(s, &s->d_tab, (size_t)T_NTAB * tp)) return 1;
+  HIPOK(hipMemcpyAsync(s->d_tab, tab, sizeof(double) * T_NTAB * tp, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  s->h_tab = tab;
+  s->h_xf = xf;
+  s->h_xc = xc;
+  #define DT(t) (s->d_tab + (size_t)(t) * tp)
+  tc_coef_t * c = &s->coef;
+  c->hxxf = DT(T_HXXF); c->hxxc = DT(T_HXXC); c->hyxf = DT(T_HYXF); c->hyxc = DT(T_HYXC);
+  c->jdxf = DT(T_JDXF); c->jdxc = DT(T_JDXC);
+  c->axf = DT(T_AXF); c->ijdxf = DT(T_IJDXF); c->ihxf = DT(T_IHXF); c->ihyf = DT(T_IHYF);
+  c->jhxf = DT(T_JHXF); c->jhyf = DT(T_JHYF); c->jhzf = DT(T_JHZF); c->l1f = DT(T_L1F);
+  c->nujf = DT(T_NUJF); c->n2jf = DT(T_N2JF); c->d1m = DT(T_D1M); c->d1p = DT(T_D1P);
+  c->ayc = DT(T_AYC); c->azc = DT(T_AZC); c->ijdxc = DT(T_IJDXC); c->ihyc = DT(T_IHYC);
+  c->jhyc = DT(T_JHYC); c->jhzc = DT(T_JHZC); c->l1c = DT(T_L1C); c->hdc = DT(T_HDC);
+  c->nujc = DT(T_NUJC); c->n2jc = DT(T_N2JC); c->dam = DT(T_DAM); c->dap = DT(T_DAP);
+  c->lxu_l = DT(T_LXU_L); c->lxu_c = DT(T_LXU_C); c->lxu_u = DT(T_LXU_U);
+  c->lyu_l = DT(T_LYU_L); c->lyu_c = DT(T_LYU_C); c->lyu_u = DT(T_LYU_U);
+  c->lxc_l = DT(T_LXC_L); c->lxc_c = DT(T_LXC_C); c->lxc_u = DT(T_LXC_U);
+  c->lyc_l = DT(T_LYC_L); c->lyc_c = DT(T_LYC_C); c->lyc_u = DT(T_LYC_U);
+  c->plx_l = DT(T_PLX_L); c->plx_u = DT(T_PLX_U); c->ply = DT(T_PLY); c->plz = DT(T_PLZ);
+  #undef DT
+  /* predict/ux.c:70-78 */
+  c->lapz_l = 1. / hz / hz;
+  c->lapz_u = 1. / hz / hz;
+  c->lapz_c = - c->lapz_l - c->lapz_u;
+  c->hz = hz;
+  c->ihz = 1. / hz;
+  c->nu = nu;
+  c->kappa = kappa;
+  #undef TB
+  return 0;
+}
+
+/* FFT twiddles exp(-2 pi i t / n), t = 0..cnt-1 */
+static int upload_twiddles(tc_solver_t * s, int n, int cnt, double ** dptr){
+  double * h = malloc(sizeof(double) * 2 * (size_t)cnt);
+  if(!h) return tc_fail("host allocation failed");
+  const double two_pi = 6.283185307179586476925286766559;
+  for(int t = 0; t < cnt; t++){
+    const double ang = - two_pi * (double)t / (double)n;
+    h[2 * t] = cos(ang);
+    h[2 * t + 1] = sin(ang);
+  }
+  if(dalloc(s, dptr, 2 * (size_t)cnt)){ free(h); return 1; }
+  hipError_t e = hipMemcpyAsync(*dptr, h, sizeof(double) * 2 * cnt, hipMemcpyHostToDevice, s->stream);
+  if(e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  free(h);
+  if(e != hipSuccess) return tc_fail("twiddle upload: %s", hipGetErrorString(e));
+  return 0;
+}
+
+static int setup_poisson(tc_solver_t * s){
+  const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+  const int nky = ny / 2 + 1;
+  s->nky = nky;
+  const double g_pi = 3.14159265358979324;
+  double * evy = malloc(sizeof(double) * nky);
+  double * evz = malloc(sizeof(double) * nz);
+  if(!evy || !evz) return tc_fail("host allocation failed");
+  for(int n = 0; n < nky; n++) evy[n] = - 4. * pow(sin(g_pi * n / ny), 2.);
+  for(int n = 0; n < nz; n++) evz[n] = - 4. * pow(sin(g_pi * n / nz), 2.);
+  if(dalloc(s, &s->d_evy, nky) || dalloc(s, &s->d_evz, nz)) return 1;
+  if(dalloc(s, &s->d_pl, nx) || dalloc(s, &s->d_pu, nx)) return 1;
+  HIPOK(hipMemcpyAsync(s->d_evy, evy, sizeof(double) * nky, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipMemcpyAsync(s->d_evz, evz, sizeof(double) * nz, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipMemcpyAsync(s->d_pl, s->h_pl, sizeof(double) * nx, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipMemcpyAsync(s->d_pu, s->h_pu, sizeof(double) * nx, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  free(evy);
+  free(evz);
+  const size_t nspec = (size_t)nx * nky * nz;
+  if(dalloc(s, &s->d_spec, 2 * nspec)) return 1;
+  if(dalloc(s, &s->d_pr, nspec) || dalloc(s, &s->d_pv, nspec)) return 1;
+  KOK(tck_launch_pfactor(nx, nky, nz, s->d_pl, s->d_pu, s->coef.hyxc, s->d_evy, s->d_evz,
+      s->hz, s->d_pr, s->d_pv, s->stream));
+  if(upload_twiddles(s, ny / 2, ny / 2, &s->d_twy)) return 1;
+  if(upload_twiddles(s, ny, ny / 2 + 1, &s->d_twyn)) return 1;
+  if(upload_twiddles(s, nz, nz, &s->d_twz)) return 1;
+  HIPOK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+/* per-field line-solve tables */
+static int setup_linesolve(tc_solver_t * s){
+  const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
+  const int tp = s->tp;
+  for(int f = 0; f < TC_NFIELD; f++){
+    tc_ls_t * L = &s->ls[f];
+    if(s->p.implicit[0]){
+      if(dalloc(s, &L->xr, nx + 1) || dalloc(s, &L->xv, nx + 1) || dalloc(s, &L->xl, nx + 1)) return 1;
+    }
+    if(s->p.implicit[1]){
+      if(dalloc(s, &L->yr, (size_t)ny * tp) || dalloc(s, &L->yv, (size_t)ny * tp) ||
+         dalloc(s, &L->yq1, (size_t)ny * tp) || dalloc(s, &L->ymeta, (size_t)3 * tp)) return 1;
+    }
+    if(s->p.implicit[2]){
+      if(dalloc(s, &L->zr, nz) || dalloc(s, &L->zv, nz) || dalloc(s, &L->zq1, nz) ||
+         dalloc(s, &L->zmeta, 3)) return 1;
+    }
+  }
+  return 0;
+}
+
+int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_solver_t ** out){
+  *out = NULL;
+  if(p->nx < 3 || p->ny < 4 || p->nz < 3) return tc_fail("grid too small: %d x %d x %d", p->nx, p->ny, p->nz);
+  if(p->ny % 2 != 0) return tc_fail("ny must be even (real theta transform)");
+  if(!tck_fft_supported(p->ny / 2) || !tck_fft_supported(p->nz))
+    return tc_fail("ny/2 = %d and nz = %d must be products of 2 and 3", p->ny / 2, p->nz);
+  if(p->nx > 1024) return tc_fail("nx = %d exceeds 1024 (radial line solver)", p->nx);
+  if(p->ny / 2 > 4096 || p->nz > 4096) return tc_fail("transform length exceeds 4096");
+  if(p->precision != TC_FP64) return tc_fail("precision %d not supported", p->precision);
+  tc_solver_t * s = calloc(1, sizeof(tc_solver_t));
+  if(!s) return tc_fail("host allocation failed");
+  s->p = *p;
+  if(hipSetDevice(p->device) != hipSuccess){
+    free(s);
+    return tc_fail("hipSetDevice(%d) failed", p->device);
+  }
+  if(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess){
+    free(s);
+    return tc_fail("hipStreamCreate failed");
+  }
+  tc_geom_t * g = &s->g;
+  g->nx = p->nx; g->ny = p->ny; g->nz = p->nz;
+  g->px = ((p->nx + 2) + 7) / 8 * 8;
+  g->py = p->ny + 2;
+  g->pz = p->nz + 2;
+  g->plane = (long long)g->px * g->py;
+  g->n3 = g->plane * g->pz;
+  s->flags.implicit[0] = p->implicit[0];
+  s->flags.implicit[1] = p->implicit[1];
+  s->flags.implicit[2] = p->implicit[2];
+  s->flags.scalar = p->scalar;
+  s->flags.uy_xm = p->uy_xm; s->flags.uy_xp = p->uy_xp;
+  s->flags.t_xm = p->t_xm; s->flags.t_xp = p->t_xp;
+  int rc = build_tables(s, xf, xc);
+  const size_t n3 = (size_t)g->n3;
+  for(int f = 0; f < TC_NFIELD && !rc; f++){
+    rc |= dalloc(s, &s->u[f][0], n3);
+    rc |= dalloc(s, &s->u[f][1], n3);
+    rc |= dalloc(s, &s->src[f][0], n3);
+    rc |= dalloc(s, &s->src[f][1], n3);
+  }
+  if(!rc) rc |= dalloc(s, &s->d_p, n3);
+  if(!rc) rc |= dalloc(s, &s->d_psi, n3);
+  if(!rc) rc |= dalloc(s, &s->d_dt, 4);
+  if(!rc) rc |= dalloc(s, (double **)&s->d_dtmin, 2);
+  if(!rc){
+    const long long nb = tck_diag_cells_blocks(g) > tck_diag_walls_blocks(g) ? tck_diag_cells_blocks(g) : tck_diag_walls_blocks(g);
+    rc |= dalloc(s, &s->d_part, (size_t)nb * 8);
+    rc |= dalloc(s, &s->d_diag, 16);
+  }
+  if(!rc) rc = setup_poisson(s);
+  if(!rc) rc = setup_linesolve(s);
+  if(!rc){
+    /* boundary values on every buffer (fields start at zero) */
+    for(int f = 0; f < TC_NFIELD && !rc; f++){
+      for(int b = 0; b < 2 && !rc; b++) rc = tc_apply_bc(s, f, s->u[f][b]);
+    }
+    if(!rc) rc = tc_apply_bc(s, TC_FP, s->d_p);
+    if(!rc) rc = tc_apply_bc(s, TC_FPSI, s->d_psi);
+  }
+  if(!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = tc_fail("initialisation failed");
+  if(rc){
+    tc_destroy(s);
+    return 1;
+  }
+  s->dt_valid = 0;
+  s->diag_valid = 0;
+  *out = s;
+  return 0;
+}
+
+int tc_destroy(tc_solver_t * s){
+  if(!s) return 0;
+  hipSetDevice(s->p.device);
+  if(s->stream) hipStreamSynchronize(s->stream);
+  for(int n = 0; n < s->nalloc; n++) hipFree(s->allocs[n]);
+  for(int n = 0; n < TC_NSTAGE; n++){
+    for(int e = 0; e < TC_MAXEV; e++){
+      if(s->prof[n].ev[e][0]){ hipEventDestroy(s->prof[n].ev[e][0]); hipEventDestroy(s->prof[n].ev[e][1]); }
+    }
+  }
+  if(s->stream) hipStreamDestroy(s->stream);
+  free(s->h_tab); free(s->h_xf); free(s->h_xc); free(s->h_pl); free(s->h_pu);
+  free(s);
+  return 0;
+}
+
+size_t tc_device_bytes(const tc_solver_t * s){
+  return s ? s->dev_bytes : 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * boundary values and host <-> device copies
+ * ------------------------------------------------------------------------------------- */
+int tc_apply_bc(tc_solver_t * s, int field, double * a){
+  int kind;
+  double wm = 0., wp = 0.;
+  switch(field){
+    case TC_FUX: kind = 0; break;
+    case TC_FUY: kind = 1; wm = s->p.uy_xm; wp = s->p.uy_xp; break;
+    case TC_FUZ: kind = 2; break;
+    case TC_FT: kind = 3; wm = s->p.t_xm; wp = s->p.t_xp; break;
+    default: kind = 4; break;
+  }
+  KOK(tck_launch_bc(&s->g, a, kind, wm, wp, s->stream));
+  return 0;
+}
+
+static int field_id(const char * name){
+  if(!strcmp(name, "ux")) return TC_FUX;
+  if(!strcmp(name, "uy")) return TC_FUY;
+  if(!strcmp(name, "uz")) return TC_FUZ;
+  if(!strcmp(name, "t")) return TC_FT;
+  if(!strcmp(name, "p")) return TC_FP;
+  if(!strcmp(name, "psi")) return TC_FPSI;
+  return -1;
+}
+
+double * tc_field_ptr(tc_solver_t * s, int f){
+  if(f >= 0 && f < TC_NFIELD) return s->u[f][s->cur[f]];
+  if(f == TC_FP) return s->d_p;
+  if(f == TC_FPSI) return s->d_psi;
+  return NULL;
+}
+
+/* NPY x extent and first device slot */
+static void npy_x(const tc_solver_t * s, int f, int * nxs, int * i0){
+  if(f == TC_FUX){ *nxs = s->p.nx + 1; *i0 = 1; }
+  else { *nxs = s->p.nx + 2; *i0 = 0; }
+}
+
+int tc_set_field(tc_solver_t * s, const char * name, const double * data){
+  const int f = field_id(name);
+  if(f < 0) return tc_fail("unknown field '%s'", name);
+  HIPOK(hipSetDevice(s->p.device));
+  int nxs, i0;
+  npy_x(s, f, &nxs, &i0);
+  const tc_geom_t * g = &s->g;
+  double * dst = tc_field_ptr(s, f);
+  for(int k = 1; k <= g->nz; k++){
+    const double * src = data + (size_t)(k - 1) * g->ny * nxs;
+    double * d = dst + (size_t)i0 + (size_t)g->px * (1 + (size_t)g->py * k);
+    HIPOK(hipMemcpy2DAsync(d, sizeof(double) * g->px, src, sizeof(double) * nxs,
+        sizeof(double) * nxs, g->ny, hipMemcpyHostToDevice, s->stream));
+  }
+  if(tc_apply_bc(s, f, dst)) return 1;
+  if(f < TC_NFIELD){
+    /* keep the second buffer's walls consistent */
+    HIPOK(hipMemcpyAsync(s->u[f][1 - s->cur[f]], dst, sizeof(double) * g->n3, hipMemcpyDeviceToDevice, s->stream));
+  }
+  HIPOK(hipStreamSynchronize(s->stream));
+  s->dt_valid = 0;
+  s->diag_valid = 0;
+  return 0;
+}
+
+int tc_get_field(tc_solver_t * s, const char * name, double * data){
+  const int f = field_id(name);
+  if(f < 0) return tc_fail("unknown field '%s'", name);
+  HIPOK(hipSetDevice(s->p.device));
+  int nxs, i0;
+  npy_x(s, f, &nxs, &i0);
+  const tc_geom_t * g = &s->g;
+  const double * src = tc_field_ptr(s, f);
+  for(int k = 1; k <= g->nz; k++){
+    double * d = data + (size_t)(k - 1) * g->ny * nxs;
+    const double * sp = src + (size_t)i0 + (size_t)g->px * (1 + (size_t)g->py * k);
+    HIPOK(hipMemcpy2DAsync(d, sizeof(double) * nxs, sp, sizeof(double) * g->px,
+        sizeof(double) * nxs, g->ny, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIPOK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * profiling: events around the kernels of each stage
+ * ------------------------------------------------------------------------------------- */
+static const char * g_stage_names[TC_NSTAGE] = {"dt", "factor", "predict", "xsolve", "ysolve", "zsolve",
+  "div_yfft", "zfft", "pxsolve", "yifft", "correct", "diag"};
+
+int tc_profile_enable(tc_solver_t * s, int on){
+  s->prof_on = on;
+  if(on){
+    for(int n = 0; n < TC_NSTAGE; n++){
+      s->prof[n].nrec = 0;
+      s->prof[n].total_ms = 0.;
+      s->prof[n].launches = 0;
+    }
+  }
+  return 0;
+}
+
+static void prof_flush(tc_solver_t * s, int n){
+  tc_prof_t * P = &s->prof[n];
+  if(P->nrec == 0) return;
+  hipEventSynchronize(P->ev[P->nrec - 1][1]);
+  for(int e = 0; e < P->nrec; e++){
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, P->ev[e][0], P->ev[e][1]);
+    P->total_ms += ms;
+    P->launches++;
+  }
+  P->nrec = 0;
+}
+
+void tc_prof_begin(tc_solver_t * s, int stage){
+  if(!s->prof_on) return;
+  tc_prof_t * P = &s->prof[stage];
+  if(P->nrec == TC_MAXEV) prof_flush(s, stage);
+  if(!P->ev[P->nrec][0]){
+    hipEventCreate(&P->ev[P->nrec][0]);
+    hipEventCreate(&P->ev[P->nrec][1]);
+  }
+  hipEventRecord(P->ev[P->nrec][0], s->stream);
+}
+
+void tc_prof_end(tc_solver_t * s, int stage){
+  if(!s->prof_on) return;
+  tc_prof_t * P = &s->prof[stage];
+  hipEventRecord(P->ev[P->nrec][1], s->stream);
+  P->nrec++;
+}
+
+int tc_profile_read(tc_solver_t * s, const char * stage, double * total_ms, long long * launches){
+  for(int n = 0; n < TC_NSTAGE; n++){
+    if(!strcmp(stage, g_stage_names[n])){
+      prof_flush(s, n);
+      *total_ms = s->prof[n].total_ms;
+      *launches = s->prof[n].launches;
+      return 0;
+    }
+  }
+  return tc_fail("unknown stage '%s'", stage);
+}
+
+#define PROF(stage, call) do { tc_prof_begin(s, stage); int e__ = (call); tc_prof_end(s, stage); \
+  if(e__) return tc_fail("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString((hipError_t)e__)); } while(0)
+
+/* ---------------------------------------------------------------------------------------
+ * the step
+ * ------------------------------------------------------------------------------------- */
+static int launch_dt(tc_solver_t * s){
+  if(s->p.dt_fixed > 0.){
+    PROF(TC_S_DT, tck_launch_dt_finalise(s->d_dtmin, s->d_dt, s->p.coef_dt_adv, s->dif_limit, s->p.dt_fixed, s->stream));
+    return 0;
+  }
+  if(!s->dt_valid){
+    PROF(TC_S_DT, tck_launch_dt(&s->g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+        tc_field_ptr(s, TC_FUZ), s->d_dtmin, s->stream));
+  }
+  PROF(TC_S_DT, tck_launch_dt_finalise(s->d_dtmin, s->d_dt, s->p.coef_dt_adv, s->dif_limit, 0., s->stream));
+  s->dt_valid = 0;
+  return 0;
+}
+
+/* stale-tensor diagnostics of the reference's logging: gradients at the start of the last
+ * predict (injection.c:49-51, dissipation.c:36-229) */
+static int snapshot_diag(tc_solver_t * s){
+  PROF(TC_S_DIAG, tck_launch_diag_walls(&s->g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+      tc_field_ptr(s, TC_FT), s->p.scalar, s->d_part, s->d_diag + 8, s->stream));
+  PROF(TC_S_DIAG, tck_launch_diag_cells(&s->g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+      tc_field_ptr(s, TC_FUZ), tc_field_ptr(s, TC_FT), s->p.scalar, s->d_part, s->d_diag, s->stream));
+  HIPOK(hipMemcpyAsync(s->h_diag_snap, s->d_diag, sizeof(double) * 10, hipMemcpyDeviceToHost, s->stream));
+  s->diag_pending = 1;
+  return 0;
+}
+
+static int predict(tc_solver_t * s, int rkstep){
+  const int nf = s->p.scalar ? 4 : 3;
+  const double diff[4] = {s->nu, s->nu, s->nu, s->kappa};
+  /* fac: 2 on the diagonal stress direction (ux.c:499-506, uy.c:539-544, uz.c:449-454)
+   * and for T in y (t.c:346-349) */
+  static const double fac[4][3] = {{2., 1., 1.}, {1., 2., 1.}, {1., 1., 2.}, {1., 2., 1.}};
+  const int any_implicit = s->p.implicit[0] || s->p.implicit[1] || s->p.implicit[2];
+  if(s->diag_armed && rkstep == 2){
+    if(snapshot_diag(s)) return 1;
+  }
+  if(any_implicit){
+    for(int f = 0; f < nf; f++){
+      tc_ls_t * L = &s->ls[f];
+      const tc_coef_t * c = &s->coef;
+      const int ux = f == TC_FUX;
+      PROF(TC_S_FACTOR, tck_launch_factor(s->d_dt, rkstep, diff[f], fac[f], s->p.implicit,
+          s->p.nx, s->p.ny, s->p.nz, ux ? 2 : 1, ux ? s->p.nx - 1 : s->p.nx,
+          ux ? c->lxu_l : c->lxc_l, ux ? c->lxu_c : c->lxc_c, ux ? c->lxu_u : c->lxc_u,
+          ux ? c->lyu_l : c->lyc_l, ux ? c->lyu_c : c->lyc_c, ux ? c->lyu_u : c->lyc_u,
+          c->lapz_l, c->lapz_c, c->lapz_u, s->tp,
+          L->xr, L->xv, L->xl, L->yr, L->yv, L->yq1, L->ymeta, L->zr, L->zv, L->zq1, L->zmeta, s->stream));
+    }
+  }
+  const double * cur[4];
+  double * nxt[4];
+  const double * srcb[4];
+  double * srca[4];
+  for(int f = 0; f < 4; f++){
+    cur[f] = s->u[f][s->cur[f]];
+    nxt[f] = s->u[f][1 - s->cur[f]];
+    srcb[f] = rkstep > 0 ? s->src[f][(rkstep - 1) & 1] : NULL;
+    srca[f] = rkstep < 2 ? s->src[f][rkstep & 1] : NULL;
+  }
+  PROF(TC_S_PREDICT, tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, srcb, srca,
+      s->d_dt, rkstep, !any_implicit, s->stream));
+  if(any_implicit){
+    int last = 0;
+    for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
+    for(int f = 0; f < nf; f++){
+      tc_ls_t * L = &s->ls[f];
+      const int ux = f == TC_FUX;
+      const int i0 = ux ? 2 : 1;
+      const int n = ux ? s->p.nx - 1 : s->p.nx;
+      if(s->p.implicit[0]){
+        PROF(TC_S_XSOLVE, tck_launch_xsolve(&s->g, nxt[f], last == 0 ? cur[f] : NULL, L->xr, L->xv, L->xl, i0, n, s->stream));
+      }
+      if(s->p.implicit[1]){
+        PROF(TC_S_YSOLVE, tck_launch_psolve(&s->g, 1, nxt[f], last == 1 ? cur[f] : NULL, L->yr, L->yv, L->yq1,
+            L->ymeta, s->tp, 1, i0, n, s->stream));
+      }
+      if(s->p.implicit[2]){
+        PROF(TC_S_ZSOLVE, tck_launch_psolve(&s->g, 2, nxt[f], cur[f], L->zr, L->zv, L->zq1, L->zmeta, 1, 0,
+            i0, n, s->stream));
+      }
+    }
+  }
+  for(int f = 0; f < nf; f++) s->cur[f] = 1 - s->cur[f];
+  return 0;
+}
+
+static int potential(tc_solver_t * s, int rkstep){
+  const tc_geom_t * g = &s->g;
+  PROF(TC_S_DIVY, tck_launch_div_yfft(g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+      tc_field_ptr(s, TC_FUZ), s->d_dt, rkstep, s->d_spec, s->d_twy, s->d_twyn, s->stream));
+  PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, -1, s->stream));
+  PROF(TC_S_PXSOLVE, tck_launch_pxsolve(g->nx, s->nky * g->nz, s->d_spec, s->d_pr, s->d_pv, s->d_pl, s->stream));
+  PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, +1, s->stream));
+  PROF(TC_S_YIFFT, tck_launch_yifft_psi(g, s->d_spec, s->d_psi, s->d_twy, s->d_twyn, s->stream));
+  return 0;
+}
+
+/* mode: 1 velocity correction (correct/*.c), 2 pressure update (update_pressure.c), 3 both */
+static int correct(tc_solver_t * s, int rkstep, int mode, int want_cfl){
+  PROF(TC_S_CORRECT, tck_launch_correct(&s->g, &s->coef, &s->flags, tc_field_ptr(s, TC_FUX),
+      tc_field_ptr(s, TC_FUY), tc_field_ptr(s, TC_FUZ), s->d_p, s->d_psi, s->d_dt, rkstep, mode,
+      want_cfl ? s->d_dtmin : NULL, s->stream));
+  return 0;
+}
+
+int tc_enqueue_step(tc_solver_t * s){
+  if(launch_dt(s)) return 1;
+  for(int r = 0; r < 3; r++){
+    if(predict(s, r)) return 1;
+    if(potential(s, r)) return 1;
+    /* the last correction also yields the CFL minimum of the next step (decide_dt.c:32-80
+     * reads exactly the velocity this kernel writes) */
+    const int cfl = (r == 2) && !(s->p.dt_fixed > 0.);
+    if(correct(s, r, 3, cfl)) return 1;
+    if(cfl) s->dt_valid = 1;
+  }
+  return 0;
+}
+
+int tc_step(tc_solver_t * s, double * dt){
+  HIPOK(hipSetDevice(s->p.device));
+  if(tc_enqueue_step(s)) return 1;
+  HIPOK(hipMemcpyAsync(&s->h_dt, s->d_dt, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  if(s->diag_pending){
+    memcpy(s->diag_vals, s->h_diag_snap, sizeof(s->diag_vals));
+    s->diag_valid = 1;
+    s->diag_pending = 0;
+  }
+  if(dt) *dt = s->h_dt;
+  return 0;
+}
+
+int tc_step_async(tc_solver_t * s, int nsteps){
+  HIPOK(hipSetDevice(s->p.device));
+  for(int n = 0; n < nsteps; n++){
+    if(tc_enqueue_step(s)) return 1;
+  }
+  return 0;
+}
+
+int tc_synchronize(tc_solver_t * s){
+  HIPOK(hipSetDevice(s->p.device));
+  HIPOK(hipStreamSynchronize(s->stream));
+  if(s->diag_pending){
+    memcpy(s->diag_vals, s->h_diag_snap, sizeof(s->diag_vals));
+    s->diag_valid = 1;
+    s->diag_pending = 0;
+  }
+  return 0;
+}
+
+int tc_get_dt(tc_solver_t * s, double * dt){
+  HIPOK(hipMemcpyAsync(&s->h_dt, s->d_dt, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  *dt = s->h_dt;
+  return 0;
+}
+
+int tc_decide_dt(tc_solver_t * s, double * dt){
+  HIPOK(hipSetDevice(s->p.device));
+  s->dt_valid = 0;
+  if(launch_dt(s)) return 1;
+  return tc_get_dt(s, dt);
+}
+
+static int set_dt(tc_solver_t * s, double dt){
+  s->h_dt = dt;
+  HIPOK(hipMemcpyAsync(s->d_dt, &s->h_dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  return 0;
+}
+
+int tc_predict(tc_solver_t * s, int rkstep, double dt){
+  if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
+  HIPOK(hipSetDevice(s->p.device));
+  if(set_dt(s, dt) || predict(s, rkstep)) return 1;
+  s->dt_valid = 0;
+  return tc_synchronize(s);
+}
+
+int tc_potential(tc_solver_t * s, int rkstep, double dt){
+  if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
+  HIPOK(hipSetDevice(s->p.device));
+  if(set_dt(s, dt) || potential(s, rkstep)) return 1;
+  return tc_synchronize(s);
+}
+
+int tc_correct(tc_solver_t * s, int rkstep, double dt){
+  if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
+  HIPOK(hipSetDevice(s->p.device));
+  if(set_dt(s, dt) || correct(s, rkstep, 1, 0)) return 1;
+  s->dt_valid = 0;
+  return tc_synchronize(s);
+}
+
+int tc_pressure(tc_solver_t * s, int rkstep, double dt){
+  if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
+  HIPOK(hipSetDevice(s->p.device));
+  if(set_dt(s, dt) || correct(s, rkstep, 2, 0)) return 1;
+  return tc_synchronize(s);
+}
+
+/* ---------------------------------------------------------------------------------------
+ * diagnostics
+ * ------------------------------------------------------------------------------------- */
+static int cells_now(tc_solver_t * s, double out[8]){
+  HIPOK(hipSetDevice(s->p.device));
+  KOK(tck_launch_diag_cells(&s->g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+      tc_field_ptr(s, TC_FUZ), tc_field_ptr(s, TC_FT), s->p.scalar, s->d_part, s->d_diag, s->stream));
+  HIPOK(hipMemcpyAsync(out, s->d_diag, sizeof(double) * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int tc_divergence(tc_solver_t * s, double * divmax){
+  double q[8];
+  if(cells_now(s, q)) return 1;
+  *divmax = q[0];
+  return 0;
+}
+
+int tc_total_energy(tc_solver_t * s, double out[4]){
+  double q[8];
+  if(cells_now(s, q)) return 1;
+  out[0] = q[3]; out[1] = q[4]; out[2] = q[5]; out[3] = q[6];
+  return 0;
+}
+
+/* values of the reference's logging: the velocity terms come from the tensors left by the
+ * last predict (snapshot), the scalar terms read the current T (injection.c:75-87,
+ * dissipation.c:243-318) */
+static int logging_values(tc_solver_t * s, double vals[10]){
+  HIPOK(hipSetDevice(s->p.device));
+  KOK(tck_launch_diag_walls(&s->g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+      tc_field_ptr(s, TC_FT), s->p.scalar, s->d_part, s->d_diag + 8, s->stream));
+  KOK(tck_launch_diag_cells(&s->g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+      tc_field_ptr(s, TC_FUZ), tc_field_ptr(s, TC_FT), s->p.scalar, s->d_part, s->d_diag, s->stream));
+  HIPOK(hipMemcpyAsync(vals, s->d_diag, sizeof(double) * 10, hipMemcpyDeviceToHost, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  if(s->diag_valid){
+    vals[1] = s->diag_vals[1];
+    vals[8] = s->diag_vals[8];
+  }
+  return 0;
+}
+
+int tc_injection(tc_solver_t * s, double out[2]){
+  double v[10];
+  if(logging_values(s, v)) return 1;
+  out[0] = s->nu * v[8];
+  out[1] = s->kappa * v[9];
+  return 0;
+}
+
+int tc_dissipation(tc_solver_t * s, double out[2]){
+  double v[10];
+  if(logging_values(s, v)) return 1;
+  out[0] = s->nu * v[1];
+  out[1] = s->kappa * v[2];
+  return 0;
+}
+
+int tc_arm_diagnostics(tc_solver_t * s, int on){
+  s->diag_armed = on;
+  return 0;
+}

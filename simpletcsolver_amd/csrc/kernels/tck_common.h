@@ -1,0 +1,75 @@
+/*
+ * tck_common.h -- device helpers shared by the HIP kernels (gfx950, wave64).
+ */
+#ifndef TCK_COMMON_H
+#define TCK_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../tc_shared.h"
+
+#define TCK_WAVE 64
+
+/* launch-bounds hint used by every kernel below; keeps register allocation predictable */
+#define TCK_BLOCK 256
+
+__device__ __forceinline__ long long tck_idx(const tc_geom_t & g, int i, int j, int k){
+  return (long long)i + (long long)g.px * ((long long)j + (long long)g.py * (long long)k);
+}
+
+/* Store an interior value (1 <= j <= ny, 1 <= k <= nz) together with its periodic halo
+ * images (src/halo.c:18-142 fills j = 0 from ny, j = ny+1 from 1, then the z planes over the
+ * full (x, y) extent, so corners are images of images).  With xcopy set the value is also
+ * written to the x wall slot next to it (Neumann copy of src/fluid/boundary/{p,psi}.c). */
+template <typename T>
+__device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i, int j, int k, T v, bool xcopy){
+  const int nj = (j == 1) ? g.ny + 1 : (j == g.ny) ? 0 : -1;
+  const int nk = (k == 1) ? g.nz + 1 : (k == g.nz) ? 0 : -1;
+  const int ni = !xcopy ? -1 : (i == 1) ? 0 : (i == g.nx) ? g.nx + 1 : -1;
+  #pragma unroll
+  for(int a0 = 0; a0 < 2; a0++){
+    const int ii = a0 ? ni : i;
+    if(ii < 0) continue;
+    a[tck_idx(g, ii, j, k)] = v;
+    if(nj >= 0) a[tck_idx(g, ii, nj, k)] = v;
+    if(nk >= 0){
+      a[tck_idx(g, ii, j, nk)] = v;
+      if(nj >= 0) a[tck_idx(g, ii, nj, nk)] = v;
+    }
+  }
+}
+
+/* Store for the second neighbour case ny == 1 or nz == 1 is not supported (ny, nz >= 3 is
+ * checked on the host). */
+
+__device__ __forceinline__ double tck_shfl_up(double v, int off){
+  return __shfl_up(v, off, TCK_WAVE);
+}
+__device__ __forceinline__ double tck_shfl_down(double v, int off){
+  return __shfl_down(v, off, TCK_WAVE);
+}
+__device__ __forceinline__ double tck_shfl(double v, int lane){
+  return __shfl(v, lane, TCK_WAVE);
+}
+
+/* order-preserving maps between non-negative doubles and their bit patterns (atomic min/max) */
+__device__ __forceinline__ unsigned long long tck_bits(double v){
+  return (unsigned long long)__double_as_longlong(v);
+}
+
+__device__ __forceinline__ double tck_wave_min(double v){
+  for(int off = TCK_WAVE / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, TCK_WAVE));
+  return v;
+}
+__device__ __forceinline__ double tck_wave_max(double v){
+  for(int off = TCK_WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, TCK_WAVE));
+  return v;
+}
+__device__ __forceinline__ double tck_wave_sum(double v){
+  for(int off = TCK_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, TCK_WAVE);
+  return v;
+}
+
+#define TCK_CHECK_LAUNCH() do { hipError_t e_ = hipGetLastError(); if(e_ != hipSuccess) return (int)e_; } while(0)
+
+#endif

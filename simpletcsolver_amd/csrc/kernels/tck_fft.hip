@@ -1,0 +1,401 @@
+/*
+ * tck_fft.hip -- batched FFTs of the Poisson solve (gfx950), replacing the FFTW plans of
+ * src/fluid/compute_potential.c:247-309 and the pencil transposes around them (:517-581).
+ *
+ * Spectrum layout in HBM: S[kz][ky][i] (complex, i fastest, row pitch nx), ky = 0..ny/2.
+ * With i innermost, the theta and z transforms read and write rows of a few consecutive i
+ * (coalesced), and the radial solve reads each (ky, kz) line contiguously: no transpose
+ * pass is needed anywhere.
+ *
+ *  tck_div_yfft   assign_input (:382-425: divergence / (gamma dt ny nz)) fused with the
+ *                 real-to-complex theta transform; a real length-N transform is done as a
+ *                 complex length-N/2 transform of (x_2m + i x_2m+1) plus the usual split.
+ *  tck_zfft       complex axial transforms, forward or backward, in place.
+ *  tck_yifft_psi  complex-to-real theta transform fused with extract_output (:427-443): psi
+ *                 with its periodic halos and Neumann wall copies (boundary/psi.c:13-45).
+ * Transforms are unnormalised, as FFTW's; the imaginary parts of the DC and Nyquist bins are
+ * ignored by the complex-to-real transform, as FFTW's c2r does with a Hermitian input.
+ *
+ * The in-LDS transform is a Stockham autosort FFT (Govindaraju et al., "High performance
+ * discrete Fourier transforms on graphics processors", SC 2008) with radix 8, 4, 3 and 2
+ * passes; each pass reads all its butterfly inputs into registers, synchronises, and
+ * writes in place.
+ */
+#include "tck_common.h"
+
+#define FFT_MAXST 16
+#define FFT_EPT 16   /* complex elements per thread per pass (block of 256 -> 4096 per block) */
+
+struct FftPlan {
+  int M;
+  int nst;
+  int rad[FFT_MAXST];
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b){
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b){ return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b){ return make_double2(a.x - b.x, a.y - b.y); }
+/* s i a */
+__device__ __forceinline__ double2 crot(double2 a, double s){ return make_double2(- s * a.y, s * a.x); }
+__device__ __forceinline__ double2 cconj(double2 a){ return make_double2(a.x, - a.y); }
+
+/* R-point DFTs, y_k = sum_n x_n exp(s 2 pi i n k / R) */
+template <int R> __device__ __forceinline__ void dft(double2 * x, double s);
+
+template <> __device__ __forceinline__ void dft<2>(double2 * x, double s){
+  (void)s;
+  const double2 a = x[0], b = x[1];
+  x[0] = cadd(a, b);
+  x[1] = csub(a, b);
+}
+
+template <> __device__ __forceinline__ void dft<4>(double2 * x, double s){
+  const double2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+  const double2 b0 = cadd(x[1], x[3]), b1 = crot(csub(x[1], x[3]), s);
+  x[0] = cadd(a0, b0);
+  x[2] = csub(a0, b0);
+  x[1] = cadd(a1, b1);
+  x[3] = csub(a1, b1);
+}
+
+template <> __device__ __forceinline__ void dft<3>(double2 * x, double s){
+  const double h = 0.86602540378443864676;   /* sqrt(3) / 2 */
+  const double2 t1 = cadd(x[1], x[2]);
+  const double2 t2 = crot(csub(x[1], x[2]), s * h);
+  const double2 m = make_double2(x[0].x - 0.5 * t1.x, x[0].y - 0.5 * t1.y);
+  x[0] = cadd(x[0], t1);
+  x[1] = cadd(m, t2);
+  x[2] = csub(m, t2);
+}
+
+template <> __device__ __forceinline__ void dft<8>(double2 * x, double s){
+  const double r = 0.70710678118654752440;   /* sqrt(1/2) */
+  double2 e[4] = {x[0], x[2], x[4], x[6]};
+  double2 o[4] = {x[1], x[3], x[5], x[7]};
+  dft<4>(e, s);
+  dft<4>(o, s);
+  /* o_k *= w8^k, w8 = r (1 + s i) */
+  o[1] = make_double2(r * (o[1].x - s * o[1].y), r * (o[1].y + s * o[1].x));
+  o[2] = crot(o[2], s);
+  o[3] = make_double2(r * (- o[3].x - s * o[3].y), r * (- o[3].y + s * o[3].x));
+  #pragma unroll
+  for(int k = 0; k < 4; k++){
+    x[k] = cadd(e[k], o[k]);
+    x[k + 4] = csub(e[k], o[k]);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void fft_stage(double2 * buf, int M, int ld, int nb, int Ns,
+    const double2 * __restrict__ tw, double s){
+  constexpr int BPT = (FFT_EPT + R - 1) / R;
+  const int nt = blockDim.x;
+  const int tid = threadIdx.x;
+  const int per = M / R;
+  const int nbf = nb * per;
+  const int twstep = M / (Ns * R);
+  double2 v[BPT][R];
+  #pragma unroll
+  for(int q = 0; q < BPT; q++){
+    const int bf = tid + q * nt;
+    if(bf < nbf){
+      const int b = bf / per;
+      const int jj = bf - b * per;
+      const int jm = jj % Ns;
+      const double2 * src = buf + b * ld + jj;
+      #pragma unroll
+      for(int r = 0; r < R; r++){
+        double2 x = src[r * per];
+        if(r > 0){
+          double2 w = tw[jm * r * twstep];
+          if(s > 0.) w.y = - w.y;
+          x = cmul(x, w);
+        }
+        v[q][r] = x;
+      }
+      dft<R>(v[q], s);
+    }
+  }
+  __syncthreads();
+  #pragma unroll
+  for(int q = 0; q < BPT; q++){
+    const int bf = tid + q * nt;
+    if(bf < nbf){
+      const int b = bf / per;
+      const int jj = bf - b * per;
+      const int jm = jj % Ns;
+      double2 * dst = buf + b * ld + (jj - jm) * R + jm;
+      #pragma unroll
+      for(int r = 0; r < R; r++) dst[r * Ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+/* nb transforms of length P.M stored at buf + b * ld; tw[t] = exp(-2 pi i t / M) */
+__device__ void lds_fft(double2 * buf, int ld, int nb, const FftPlan & P,
+    const double2 * __restrict__ tw, double s){
+  int Ns = 1;
+  for(int st = 0; st < P.nst; st++){
+    const int R = P.rad[st];
+    if(R == 8) fft_stage<8>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(R == 4) fft_stage<4>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(R == 3) fft_stage<3>(buf, P.M, ld, nb, Ns, tw, s);
+    else fft_stage<2>(buf, P.M, ld, nb, Ns, tw, s);
+    Ns *= R;
+  }
+}
+
+__constant__ double c_fft_rka[3] = {32. / 60., 25. / 60., 45. / 60.};
+__constant__ double c_fft_rkb[3] = { 0. / 60., -17. / 60., -25. / 60.};
+
+/* ------------------------------------------------------------------------------------- */
+struct YFwdArgs {
+  tc_geom_t g;
+  tc_coef_t c;
+  const double * ux; const double * uy; const double * uz;
+  const double * dt;
+  int rkstep;
+  double2 * S;
+  int nky, ti;
+  FftPlan P;                     /* length M = ny / 2 */
+  const double2 * twM;           /* exp(-2 pi i t / M) */
+  const double2 * twN;           /* exp(-2 pi i k / ny), k = 0..M */
+};
+
+__global__ void __launch_bounds__(TCK_BLOCK) tck_div_yfft(const YFwdArgs A){
+  extern __shared__ double2 sm[];
+  const tc_geom_t & g = A.g;
+  const tc_coef_t & c = A.c;
+  const int ti = A.ti;
+  const int M = A.P.M;
+  const int i0 = 1 + blockIdx.x * ti;
+  const int k = 1 + blockIdx.y;
+  const double dt = *A.dt;
+  const double gamma = c_fft_rka[A.rkstep] + c_fft_rkb[A.rkstep];
+  const double norm = 1. * g.ny * g.nz;
+  const double prefactor = 1. / (gamma * dt) / norm;
+  const long long sy = g.px;
+  const long long sz = g.plane;
+  /* divergence (assign_input :404-423) packed as z_m = x_2m + i x_2m+1 */
+  for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
+    const int t = e % ti;
+    const int j0 = e / ti;
+    const int i = i0 + t;
+    double val = 0.;
+    if(i <= g.nx){
+      const long long o = tck_idx(g, i, j0 + 1, k);
+      const double div = c.ijdxc[i] * (
+          - c.jhxf[i] * A.ux[o] + c.jhxf[i + 1] * A.ux[o + 1]
+          - c.jhyc[i] * A.uy[o] + c.jhyc[i] * A.uy[o + sy]
+          - c.jhzc[i] * A.uz[o] + c.jhzc[i] * A.uz[o + sz]);
+      val = prefactor * div;
+    }
+    ((double *)(sm + t * M))[j0] = val;
+  }
+  __syncthreads();
+  lds_fft(sm, M, ti, A.P, A.twM, -1.);
+  /* split: X_k = (Z_k + conj Z_{M-k}) / 2 - i W^k (Z_k - conj Z_{M-k}) / 2, k = 0..M */
+  for(int e = threadIdx.x; e < ti * (M + 1); e += blockDim.x){
+    const int t = e % ti;
+    const int kk = e / ti;
+    const int i = i0 + t;
+    if(i > g.nx) continue;
+    const double2 zk = sm[t * M + (kk == M ? 0 : kk)];
+    const double2 zc = cconj(sm[t * M + (kk == 0 ? 0 : M - kk)]);
+    const double2 w = A.twN[kk];
+    const double2 ev = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+    const double2 od = make_double2(0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y));
+    /* - i w od */
+    const double2 wo = cmul(w, od);
+    const double2 X = make_double2(ev.x + wo.y, ev.y - wo.x);
+    A.S[(long long)(i - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))] = X;
+  }
+}
+
+/* ------------------------------------------------------------------------------------- */
+struct ZArgs {
+  int nx, nky, nz, tz;
+  double2 * S;
+  FftPlan P;               /* length nz */
+  const double2 * tw;
+  double sign;
+};
+
+__global__ void __launch_bounds__(TCK_BLOCK) tck_zfft(const ZArgs A){
+  extern __shared__ double2 sm[];
+  const int tz = A.tz;
+  const int nz = A.nz;
+  const int i0 = blockIdx.x * tz;
+  const int ky = blockIdx.y;
+  const long long kstride = (long long)A.nx * A.nky;
+  double2 * base = A.S + (long long)A.nx * ky;
+  for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
+    const int t = e % tz;
+    const int kz = e / tz;
+    const int ii = i0 + t;
+    sm[t * nz + kz] = ii < A.nx ? base[ii + kstride * kz] : make_double2(0., 0.);
+  }
+  __syncthreads();
+  lds_fft(sm, nz, tz, A.P, A.tw, A.sign);
+  for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
+    const int t = e % tz;
+    const int kz = e / tz;
+    const int ii = i0 + t;
+    if(ii < A.nx) base[ii + kstride * kz] = sm[t * nz + kz];
+  }
+}
+
+/* ------------------------------------------------------------------------------------- */
+struct YInvArgs {
+  tc_geom_t g;
+  const double2 * S;
+  double * psi;
+  int nky, ti;
+  FftPlan P;
+  const double2 * twM;
+  const double2 * twN;
+};
+
+/* Z'_k = (X_k + conj X_{M-k}) + i W^{-k} (X_k - conj X_{M-k}) so that
+ * IFFT_M(Z')_m = x_2m + i x_2m+1 (unnormalised, as FFTW's c2r) */
+__device__ __forceinline__ double2 c2r_pre(double2 xk, double2 xmk, double2 w){
+  const double2 xc = cconj(xmk);
+  const double2 a = cadd(xk, xc);
+  const double2 d = csub(xk, xc);
+  const double2 wd = cmul(cconj(w), d);
+  return make_double2(a.x - wd.y, a.y + wd.x);
+}
+
+__global__ void __launch_bounds__(TCK_BLOCK) tck_yifft_psi(const YInvArgs A){
+  extern __shared__ double2 sm[];
+  const tc_geom_t & g = A.g;
+  const int ti = A.ti;
+  const int M = A.P.M;
+  const int i0 = 1 + blockIdx.x * ti;
+  const int k = 1 + blockIdx.y;
+  const long long rowbase = (long long)g.nx * (long long)A.nky * (k - 1);
+  /* load X_1..X_{M-1}; Z'_0 needs X_0 and X_M only and is formed here */
+  for(int e = threadIdx.x; e < ti * M; e += blockDim.x){
+    const int t = e % ti;
+    const int kk = e / ti;
+    const int i = i0 + t;
+    double2 v = make_double2(0., 0.);
+    if(i <= g.nx){
+      const long long o = (long long)(i - 1) + rowbase;
+      if(kk == 0){
+        double2 x0 = A.S[o];
+        double2 xm = A.S[o + (long long)g.nx * M];
+        x0.y = 0.;
+        xm.y = 0.;
+        v = c2r_pre(x0, xm, A.twN[0]);
+      }else{
+        v = A.S[o + (long long)g.nx * kk];
+      }
+    }
+    sm[t * M + kk] = v;
+  }
+  __syncthreads();
+  /* pairs (k, M-k), k = 1..M/2, in place */
+  const int half = M / 2;
+  for(int e = threadIdx.x; e < ti * half; e += blockDim.x){
+    const int t = e % ti;
+    const int k1 = 1 + e / ti;
+    const int k2 = M - k1;
+    const double2 xa = sm[t * M + k1];
+    const double2 xb = sm[t * M + k2];
+    const double2 z1 = c2r_pre(xa, xb, A.twN[k1]);
+    if(k2 != k1){
+      const double2 z2 = c2r_pre(xb, xa, A.twN[k2]);
+      sm[t * M + k2] = z2;
+    }
+    sm[t * M + k1] = z1;
+  }
+  __syncthreads();
+  lds_fft(sm, M, ti, A.P, A.twM, 1.);
+  for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
+    const int t = e % ti;
+    const int jj = e / ti;
+    const int i = i0 + t;
+    if(i > g.nx) continue;
+    const double v = ((const double *)(sm + t * M))[jj];
+    tck_store_halo(A.psi, g, i, jj + 1, k, v, true);
+  }
+}
+
+/* ------------------------------------------------------------------------------------- */
+extern "C" {
+
+static int make_plan(int M, FftPlan * P){
+  P->M = M;
+  P->nst = 0;
+  int n = M;
+  while(n > 1){
+    int r;
+    if(n % 8 == 0) r = 8;
+    else if(n % 4 == 0) r = 4;
+    else if(n % 2 == 0) r = 2;
+    else if(n % 3 == 0) r = 3;
+    else return -1;
+    if(P->nst >= FFT_MAXST) return -1;
+    P->rad[P->nst++] = r;
+    n /= r;
+  }
+  return 0;
+}
+
+int tck_fft_supported(int n){
+  FftPlan P;
+  return make_plan(n, &P) == 0;
+}
+
+/* batch size per block so that batch * M <= 256 * FFT_EPT and fits 64 KiB of LDS */
+int tck_fft_batch(int M, int cap){
+  int b = 1;
+  while(b * 2 <= cap && (b * 2) * M <= TCK_BLOCK * FFT_EPT) b *= 2;
+  return b;
+}
+
+int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
+    const double * uy, const double * uz, const double * dt, int rkstep, double * S,
+    const double * twM, const double * twN, hipStream_t stream){
+  YFwdArgs A;
+  A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.uz = uz; A.dt = dt; A.rkstep = rkstep;
+  A.S = (double2 *)S; A.nky = g->ny / 2 + 1;
+  if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
+  A.ti = tck_fft_batch(A.P.M, 64);
+  A.twM = (const double2 *)twM; A.twN = (const double2 *)twN;
+  const size_t lds = sizeof(double2) * (size_t)A.ti * A.P.M;
+  hipLaunchKernelGGL(tck_div_yfft, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(TCK_BLOCK), lds, stream, A);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
+int tck_launch_zfft(int nx, int nky, int nz, double * S, const double * tw, int sign, hipStream_t stream){
+  ZArgs A;
+  A.nx = nx; A.nky = nky; A.nz = nz; A.S = (double2 *)S; A.tw = (const double2 *)tw;
+  A.sign = sign < 0 ? -1. : 1.;
+  if(make_plan(nz, &A.P) != 0) return (int)hipErrorInvalidValue;
+  A.tz = tck_fft_batch(nz, 64);
+  const size_t lds = sizeof(double2) * (size_t)A.tz * nz;
+  hipLaunchKernelGGL(tck_zfft, dim3((nx + A.tz - 1) / A.tz, nky), dim3(TCK_BLOCK), lds, stream, A);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
+int tck_launch_yifft_psi(const tc_geom_t * g, const double * S, double * psi, const double * twM,
+    const double * twN, hipStream_t stream){
+  YInvArgs A;
+  A.g = *g; A.S = (const double2 *)S; A.psi = psi; A.nky = g->ny / 2 + 1;
+  if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
+  A.ti = tck_fft_batch(A.P.M, 64);
+  A.twM = (const double2 *)twM; A.twN = (const double2 *)twN;
+  const size_t lds = sizeof(double2) * (size_t)A.ti * A.P.M;
+  hipLaunchKernelGGL(tck_yifft_psi, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(TCK_BLOCK), lds, stream, A);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
+}
