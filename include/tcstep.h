@@ -88,6 +88,8 @@ int tc_arm_diagnostics(tc_solver_t * s, int on);
 int tc_profile_enable(tc_solver_t * s, int on);
 int tc_profile_read(tc_solver_t * s, const char * stage, double * total_ms, long long * launches);
 size_t tc_device_bytes(const tc_solver_t * s);
+/* measured copy bandwidth (GB/s, read + write bytes) over nbytes, iters copies */
+int tc_probe_copy_bandwidth(tc_solver_t * s, size_t nbytes, int iters, double * gbps);
 
 /* seeded initial condition (tests/icgen.py states the same definition in numpy):
  * tanh-clustered grid (stretch s, s = 0 uniform) and laminar Couette + uniform noise */

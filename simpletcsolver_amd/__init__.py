@@ -23,7 +23,7 @@ EXPORTED = (
     "tc_decide_dt", "tc_predict", "tc_potential", "tc_correct", "tc_pressure",
     "tc_divergence", "tc_injection", "tc_dissipation", "tc_total_energy",
     "tc_profile_enable", "tc_profile_read", "tc_device_bytes", "tc_ic_grid", "tc_ic_fields",
-    "tc_arm_diagnostics",
+    "tc_arm_diagnostics", "tc_probe_copy_bandwidth",
 )
 
 
@@ -78,6 +78,7 @@ def load_library(path=None):
     lib.tc_profile_enable.argtypes = [vp, ctypes.c_int]
     lib.tc_arm_diagnostics.argtypes = [vp, ctypes.c_int]
     lib.tc_profile_read.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_longlong)]
+    lib.tc_probe_copy_bandwidth.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, dp]
     lib.tc_device_bytes.restype = ctypes.c_size_t
     lib.tc_device_bytes.argtypes = [vp]
     lib.tc_ic_grid.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, dp, dp]
@@ -239,6 +240,13 @@ class Solver:
         self._c(self._lib.tc_profile_read(self._h, stage.encode(), ctypes.byref(ms), ctypes.byref(n)),
                 "tc_profile_read")
         return ms.value, n.value
+
+    def copy_bandwidth(self, nbytes=2 << 30, iters=10):
+        """measured HBM copy bandwidth in GB/s (read + write bytes)"""
+        v = ctypes.c_double(0.0)
+        self._c(self._lib.tc_probe_copy_bandwidth(self._h, int(nbytes), int(iters), ctypes.byref(v)),
+                "tc_probe_copy_bandwidth")
+        return v.value
 
     def device_bytes(self):
         return self._lib.tc_device_bytes(self._h)

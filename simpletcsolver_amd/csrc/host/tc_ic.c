@@ -81,6 +81,13 @@ static void mode_table(unsigned long long seed, unsigned field, unsigned dir, in
   }
 }
 
+int tck_launch_ic_fill(const tc_geom_t * g, double * dst, const double * prof, const double * mx,
+    const double * my, const double * mz, double amp, double mode_amp, unsigned long long seed,
+    unsigned field, int nxs, int i0, hipStream_t stream);
+
+/* The per-index tables (profile, mode factors) are evaluated here; the device kernel
+ * tck_ic_fill combines them with the splitmix64 noise in the same operand order as the
+ * loop below would, so the field equals the one a host-side generator writes. */
 int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp){
   const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
   const int curved = s->p.is_curved;
@@ -88,33 +95,30 @@ int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_
   const double * xc = s->h_xc;
   const double ri = xf[1], ro = xf[nx + 1];
   const double lx = ro - ri;
-  const char * names[5] = {"ux", "uy", "uz", "t", "p"};
-  const size_t plane_max = (size_t)ny * (nx + 2);
-  double * full = NULL;
-  const size_t nfull = (size_t)nz * plane_max;
-  full = malloc(sizeof(double) * nfull);
-  if(!full) return tc_fail("host allocation failed (%zu MB)", sizeof(double) * nfull >> 20);
-  double * X = malloc(sizeof(double) * (nx + 2));
-  double * Y = malloc(sizeof(double) * ny);
-  double * Z = malloc(sizeof(double) * nz);
-  double * pos = malloc(sizeof(double) * ((nx + 2) > ny ? ((nx + 2) > nz ? (nx + 2) : nz) : (ny > nz ? ny : nz)));
-  if(!X || !Y || !Z || !pos){ free(full); free(X); free(Y); free(Z); free(pos); return tc_fail("host allocation failed"); }
+  const int nmax = (nx + 2) > ny ? ((nx + 2) > nz ? (nx + 2) : nz) : (ny > nz ? ny : nz);
+  double * h = calloc((size_t)4 * nmax, sizeof(double));
+  double * pos = calloc((size_t)nmax, sizeof(double));
+  double * d = NULL;
+  if(!h || !pos){ free(h); free(pos); return tc_fail("host allocation failed"); }
+  if(hipMalloc((void **)&d, sizeof(double) * 4 * nmax) != hipSuccess){
+    free(h); free(pos);
+    return tc_fail("hipMalloc failed");
+  }
+  double * prof = h, * MX = h + nmax, * Y = h + 2 * nmax, * Z = h + 3 * nmax;
   int rc = 0;
+  static const int fmap[5] = {TC_FUX, TC_FUY, TC_FUZ, TC_FT, TC_FP};
   for(int f = 0; f < 5 && !rc; f++){
     const int nxs = f == 0 ? nx + 1 : nx + 2;
-    const int fid = f == 4 ? 4 : f;   /* noise stream: ux 0, uy 1, uz 2, t 3 */
-    /* laminar profile per x index */
+    const int i0 = f == 0 ? 1 : 0;
     for(int q = 0; q < nxs; q++){
       const double r = f == 0 ? xf[q + 1] : xc[q];
       double v = 0.;
       if(f == 1) v = laminar_uy(r, ri, ro, s->p.uy_xm, s->p.uy_xp, curved);
       if(f == 3) v = conduction_t(r, ri, ro, s->p.t_xm, s->p.t_xp, curved);
-      X[q] = v;
+      prof[q] = v;
     }
     const int use_modes = nmodes > 0 && mode_amp != 0. && f < 3;
-    double * MX = NULL;
     if(use_modes){
-      MX = malloc(sizeof(double) * nxs);
       for(int q = 0; q < nxs; q++) pos[q] = ((f == 0 ? xf[q + 1] : xc[q]) - ri) / lx;
       mode_table(seed, f, 0, nmodes, 0, nxs, pos, MX);
       for(int q = 0; q < ny; q++) pos[q] = (double)q / ny;
@@ -123,21 +127,18 @@ int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_
       mode_table(seed, f, 2, nmodes, 1, nz, pos, Z);
     }
     const double a = f < 3 ? amp : (f == 3 ? t_amp : 0.);
-    for(int k = 0; k < nz; k++){
-      for(int j = 0; j < ny; j++){
-        double * row = full + ((size_t)k * ny + j) * nxs;
-        const unsigned long long lin0 = ((unsigned long long)k * ny + j) * nxs;
-        for(int q = 0; q < nxs; q++){
-          double v = X[q];
-          if(a != 0.) v = v + a * (2. * uniform01(seed, fid, lin0 + q) - 1.);
-          if(use_modes) v = v + mode_amp * MX[q] * Y[j] * Z[k];
-          row[q] = v;
-        }
-      }
+    double * dst = tc_field_ptr(s, fmap[f]);
+    if(hipMemcpyAsync(d, h, sizeof(double) * 4 * nmax, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+       tck_launch_ic_fill(&s->g, dst, d, use_modes ? d + nmax : NULL, d + 2 * nmax, d + 3 * nmax, a,
+         mode_amp, seed, (unsigned)f, nxs, i0, s->stream) != 0 ||
+       hipStreamSynchronize(s->stream) != hipSuccess){
+      rc = tc_fail("initial condition fill failed");
+      break;
     }
-    free(MX);
-    rc = tc_set_field(s, names[f], full);
+    rc = tc_finish_field(s, fmap[f]);
   }
-  free(full); free(X); free(Y); free(Z); free(pos);
+  hipFree(d);
+  free(h);
+  free(pos);
   return rc;
 }

@@ -14,7 +14,7 @@ enum { TC_FUX = 0, TC_FUY = 1, TC_FUZ = 2, TC_FT = 3, TC_FP = 4, TC_FPSI = 5 };
 enum { TC_S_DT, TC_S_FACTOR, TC_S_PREDICT, TC_S_XSOLVE, TC_S_YSOLVE, TC_S_ZSOLVE,
   TC_S_DIVY, TC_S_ZFFT, TC_S_PXSOLVE, TC_S_YIFFT, TC_S_CORRECT, TC_S_DIAG, TC_NSTAGE };
 
-#define TC_MAXALLOC 128
+#define TC_MAXALLOC 512
 #define TC_MAXEV 64
 
 /* line-solve factor tables of one field (see tck_linesolve.hip) */
@@ -56,7 +56,7 @@ struct tc_solver_s {
   unsigned long long * d_dtmin;
   double * d_part;
   double * d_diag;
-  tc_ls_t ls[TC_NFIELD];
+  tc_ls_t ls[TC_NFIELD][3];   /* per field and RK substep */
   /* host copies */
   double * h_tab; double * h_xf; double * h_xc; double * h_pl; double * h_pu;
   double h_dt;
@@ -76,6 +76,7 @@ int tc_fail(const char * fmt, ...);
 int tc_apply_bc(tc_solver_t * s, int field, double * a);
 double * tc_field_ptr(tc_solver_t * s, int f);
 int tc_enqueue_step(tc_solver_t * s);
+int tc_finish_field(tc_solver_t * s, int f);
 
 void tc_prof_begin(tc_solver_t * s, int stage);
 void tc_prof_end(tc_solver_t * s, int stage);

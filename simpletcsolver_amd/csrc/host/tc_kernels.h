@@ -22,14 +22,11 @@ int tck_launch_dt_finalise(unsigned long long * dtmin, double * dt, double coef_
     double dif_limit, double dt_fixed, hipStream_t stream);
 
 /* tck_linesolve.hip */
-int tck_launch_factor(const double * dt, int rkstep, double diff, const double fac[3],
-    const int dirs[3], int nx, int ny, int nz, int i0, int n,
-    const double * lxl, const double * lxc, const double * lxu,
-    const double * lyl, const double * lyc, const double * lyu,
-    double lzl, double lzc, double lzu, int tp,
-    double * xr, double * xv, double * xl,
-    double * yr, double * yv, double * yq1, double * ymeta,
-    double * zr, double * zv, double * zq1, double * zmeta, hipStream_t stream);
+int tck_launch_factor(const double * dt, const int dirs[3], int nx, int ny, int nz, int tp,
+    double lzl, double lzc, double lzu, int njobs, const int * rkstep, const double * diff,
+    const double (*fac)[3], const int * i0, const int * n,
+    const double * const (*lx)[3], const double * const (*ly)[3],
+    double * const (*xt)[3], double * const (*yt)[4], double * const (*zt)[4], hipStream_t stream);
 int tck_launch_xsolve(const tc_geom_t * g, double * d, const double * ucur, const double * r,
     const double * v, const double * l, int i0, int n, hipStream_t stream);
 int tck_launch_psolve(const tc_geom_t * g, int dir, double * d, const double * ucur,
@@ -51,6 +48,7 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const double * S, double * psi, co
     const double * twN, hipStream_t stream);
 
 /* tck_diag.hip */
+int tck_launch_copy(const double * src, double * dst, long long n, hipStream_t stream);
 int tck_launch_bc(const tc_geom_t * g, double * a, int kind, double wm, double wp, hipStream_t stream);
 long long tck_diag_cells_blocks(const tc_geom_t * g);
 long long tck_diag_walls_blocks(const tc_geom_t * g);

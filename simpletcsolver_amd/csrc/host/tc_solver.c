@@ -313,22 +313,24 @@ static int setup_poisson(tc_solver_t * s){
   return 0;
 }
 
-/* per-field line-solve tables */
+/* per-field, per-substep line-solve tables */
 static int setup_linesolve(tc_solver_t * s){
   const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
   const int tp = s->tp;
   for(int f = 0; f < TC_NFIELD; f++){
-    tc_ls_t * L = &s->ls[f];
-    if(s->p.implicit[0]){
-      if(dalloc(s, &L->xr, nx + 1) || dalloc(s, &L->xv, nx + 1) || dalloc(s, &L->xl, nx + 1)) return 1;
-    }
-    if(s->p.implicit[1]){
-      if(dalloc(s, &L->yr, (size_t)ny * tp) || dalloc(s, &L->yv, (size_t)ny * tp) ||
-         dalloc(s, &L->yq1, (size_t)ny * tp) || dalloc(s, &L->ymeta, (size_t)3 * tp)) return 1;
-    }
-    if(s->p.implicit[2]){
-      if(dalloc(s, &L->zr, nz) || dalloc(s, &L->zv, nz) || dalloc(s, &L->zq1, nz) ||
-         dalloc(s, &L->zmeta, 3)) return 1;
+    for(int r = 0; r < 3; r++){
+      tc_ls_t * L = &s->ls[f][r];
+      if(s->p.implicit[0]){
+        if(dalloc(s, &L->xr, nx + 1) || dalloc(s, &L->xv, nx + 1) || dalloc(s, &L->xl, nx + 1)) return 1;
+      }
+      if(s->p.implicit[1]){
+        if(dalloc(s, &L->yr, (size_t)ny * tp) || dalloc(s, &L->yv, (size_t)ny * tp) ||
+           dalloc(s, &L->yq1, (size_t)ny * tp) || dalloc(s, &L->ymeta, (size_t)3 * tp)) return 1;
+      }
+      if(s->p.implicit[2]){
+        if(dalloc(s, &L->zr, nz) || dalloc(s, &L->zv, nz) || dalloc(s, &L->zq1, nz) ||
+           dalloc(s, &L->zmeta, 3)) return 1;
+      }
     }
   }
   return 0;
@@ -356,7 +358,7 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
   }
   tc_geom_t * g = &s->g;
   g->nx = p->nx; g->ny = p->ny; g->nz = p->nz;
-  g->px = ((p->nx + 2) + 7) / 8 * 8;
+  g->px = ((p->nx + 2 + TC_XOFF) + 15) / 16 * 16;
   g->py = p->ny + 2;
   g->pz = p->nz + 2;
   g->plane = (long long)g->px * g->py;
@@ -475,13 +477,20 @@ int tc_set_field(tc_solver_t * s, const char * name, const double * data){
   double * dst = tc_field_ptr(s, f);
   for(int k = 1; k <= g->nz; k++){
     const double * src = data + (size_t)(k - 1) * g->ny * nxs;
-    double * d = dst + (size_t)i0 + (size_t)g->px * (1 + (size_t)g->py * k);
+    double * d = dst + (size_t)(i0 + TC_XOFF) + (size_t)g->px * (1 + (size_t)g->py * k);
     HIPOK(hipMemcpy2DAsync(d, sizeof(double) * g->px, src, sizeof(double) * nxs,
         sizeof(double) * nxs, g->ny, hipMemcpyHostToDevice, s->stream));
   }
+  return tc_finish_field(s, f);
+}
+
+/* after the interior of a field was replaced: boundary values and halos, the second buffer
+ * of a double-buffered field, and invalidation of what was derived from the old field */
+int tc_finish_field(tc_solver_t * s, int f){
+  const tc_geom_t * g = &s->g;
+  double * dst = tc_field_ptr(s, f);
   if(tc_apply_bc(s, f, dst)) return 1;
   if(f < TC_NFIELD){
-    /* keep the second buffer's walls consistent */
     HIPOK(hipMemcpyAsync(s->u[f][1 - s->cur[f]], dst, sizeof(double) * g->n3, hipMemcpyDeviceToDevice, s->stream));
   }
   HIPOK(hipStreamSynchronize(s->stream));
@@ -500,7 +509,7 @@ int tc_get_field(tc_solver_t * s, const char * name, double * data){
   const double * src = tc_field_ptr(s, f);
   for(int k = 1; k <= g->nz; k++){
     double * d = data + (size_t)(k - 1) * g->ny * nxs;
-    const double * sp = src + (size_t)i0 + (size_t)g->px * (1 + (size_t)g->py * k);
+    const double * sp = src + (size_t)(i0 + TC_XOFF) + (size_t)g->px * (1 + (size_t)g->py * k);
     HIPOK(hipMemcpy2DAsync(d, sizeof(double) * nxs, sp, sizeof(double) * g->px,
         sizeof(double) * nxs, g->ny, hipMemcpyDeviceToHost, s->stream));
   }
@@ -601,28 +610,52 @@ static int snapshot_diag(tc_solver_t * s){
   return 0;
 }
 
-static int predict(tc_solver_t * s, int rkstep){
+/* LU factors of the implicit-diffusion systems of substeps r0..r1 for every advected field,
+ * from the dt on the device (one launch) */
+static int factor_tables(tc_solver_t * s, int r0, int r1){
+  if(!(s->p.implicit[0] || s->p.implicit[1] || s->p.implicit[2])) return 0;
   const int nf = s->p.scalar ? 4 : 3;
-  const double diff[4] = {s->nu, s->nu, s->nu, s->kappa};
   /* fac: 2 on the diagonal stress direction (ux.c:499-506, uy.c:539-544, uz.c:449-454)
    * and for T in y (t.c:346-349) */
-  static const double fac[4][3] = {{2., 1., 1.}, {1., 2., 1.}, {1., 1., 2.}, {1., 2., 1.}};
+  static const double facs[4][3] = {{2., 1., 1.}, {1., 2., 1.}, {1., 1., 2.}, {1., 2., 1.}};
+  int rk[12], i0[12], n[12];
+  double diff[12], fac[12][3];
+  const double * lx[12][3];
+  const double * ly[12][3];
+  double * xt[12][3];
+  double * yt[12][4];
+  double * zt[12][4];
+  const tc_coef_t * c = &s->coef;
+  int nj = 0;
+  for(int r = r0; r <= r1; r++){
+    for(int f = 0; f < nf; f++){
+      const int ux = f == TC_FUX;
+      const tc_ls_t * L = &s->ls[f][r];
+      rk[nj] = r;
+      diff[nj] = f == TC_FT ? s->kappa : s->nu;
+      for(int d = 0; d < 3; d++) fac[nj][d] = facs[f][d];
+      i0[nj] = ux ? 2 : 1;
+      n[nj] = ux ? s->p.nx - 1 : s->p.nx;
+      lx[nj][0] = ux ? c->lxu_l : c->lxc_l; lx[nj][1] = ux ? c->lxu_c : c->lxc_c; lx[nj][2] = ux ? c->lxu_u : c->lxc_u;
+      ly[nj][0] = ux ? c->lyu_l : c->lyc_l; ly[nj][1] = ux ? c->lyu_c : c->lyc_c; ly[nj][2] = ux ? c->lyu_u : c->lyc_u;
+      xt[nj][0] = L->xr; xt[nj][1] = L->xv; xt[nj][2] = L->xl;
+      yt[nj][0] = L->yr; yt[nj][1] = L->yv; yt[nj][2] = L->yq1; yt[nj][3] = L->ymeta;
+      zt[nj][0] = L->zr; zt[nj][1] = L->zv; zt[nj][2] = L->zq1; zt[nj][3] = L->zmeta;
+      nj++;
+    }
+  }
+  PROF(TC_S_FACTOR, tck_launch_factor(s->d_dt, s->p.implicit, s->p.nx, s->p.ny, s->p.nz, s->tp,
+      c->lapz_l, c->lapz_c, c->lapz_u, nj, rk, diff, (const double (*)[3])fac, i0, n,
+      (const double * const (*)[3])lx, (const double * const (*)[3])ly,
+      (double * const (*)[3])xt, (double * const (*)[4])yt, (double * const (*)[4])zt, s->stream));
+  return 0;
+}
+
+static int predict(tc_solver_t * s, int rkstep){
+  const int nf = s->p.scalar ? 4 : 3;
   const int any_implicit = s->p.implicit[0] || s->p.implicit[1] || s->p.implicit[2];
   if(s->diag_armed && rkstep == 2){
     if(snapshot_diag(s)) return 1;
-  }
-  if(any_implicit){
-    for(int f = 0; f < nf; f++){
-      tc_ls_t * L = &s->ls[f];
-      const tc_coef_t * c = &s->coef;
-      const int ux = f == TC_FUX;
-      PROF(TC_S_FACTOR, tck_launch_factor(s->d_dt, rkstep, diff[f], fac[f], s->p.implicit,
-          s->p.nx, s->p.ny, s->p.nz, ux ? 2 : 1, ux ? s->p.nx - 1 : s->p.nx,
-          ux ? c->lxu_l : c->lxc_l, ux ? c->lxu_c : c->lxc_c, ux ? c->lxu_u : c->lxc_u,
-          ux ? c->lyu_l : c->lyc_l, ux ? c->lyu_c : c->lyc_c, ux ? c->lyu_u : c->lyc_u,
-          c->lapz_l, c->lapz_c, c->lapz_u, s->tp,
-          L->xr, L->xv, L->xl, L->yr, L->yv, L->yq1, L->ymeta, L->zr, L->zv, L->zq1, L->zmeta, s->stream));
-    }
   }
   const double * cur[4];
   double * nxt[4];
@@ -640,7 +673,7 @@ static int predict(tc_solver_t * s, int rkstep){
     int last = 0;
     for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
     for(int f = 0; f < nf; f++){
-      tc_ls_t * L = &s->ls[f];
+      tc_ls_t * L = &s->ls[f][rkstep];
       const int ux = f == TC_FUX;
       const int i0 = ux ? 2 : 1;
       const int n = ux ? s->p.nx - 1 : s->p.nx;
@@ -682,6 +715,7 @@ static int correct(tc_solver_t * s, int rkstep, int mode, int want_cfl){
 
 int tc_enqueue_step(tc_solver_t * s){
   if(launch_dt(s)) return 1;
+  if(factor_tables(s, 0, 2)) return 1;
   for(int r = 0; r < 3; r++){
     if(predict(s, r)) return 1;
     if(potential(s, r)) return 1;
@@ -750,7 +784,7 @@ static int set_dt(tc_solver_t * s, double dt){
 int tc_predict(tc_solver_t * s, int rkstep, double dt){
   if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
   HIPOK(hipSetDevice(s->p.device));
-  if(set_dt(s, dt) || predict(s, rkstep)) return 1;
+  if(set_dt(s, dt) || factor_tables(s, rkstep, rkstep) || predict(s, rkstep)) return 1;
   s->dt_valid = 0;
   return tc_synchronize(s);
 }
@@ -839,5 +873,33 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
 
 int tc_arm_diagnostics(tc_solver_t * s, int on){
   s->diag_armed = on;
+  return 0;
+}
+
+/* measured HBM ceiling: a device copy of nbytes, iters times, timed with events on the
+ * solver's stream; bytes moved = 2 nbytes per copy (read + write) */
+int tc_probe_copy_bandwidth(tc_solver_t * s, size_t nbytes, int iters, double * gbps){
+  HIPOK(hipSetDevice(s->p.device));
+  double * a = NULL;
+  double * b = NULL;
+  HIPOK(hipMalloc((void **)&a, nbytes));
+  if(hipMalloc((void **)&b, nbytes) != hipSuccess){ hipFree(a); return tc_fail("hipMalloc failed"); }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const long long n = (long long)(nbytes / sizeof(double));
+  int rc = tck_launch_copy(a, b, n, s->stream);
+  hipEventRecord(e0, s->stream);
+  for(int it = 0; it < iters && !rc; it++) rc = tck_launch_copy(it & 1 ? b : a, it & 1 ? a : b, n, s->stream);
+  hipEventRecord(e1, s->stream);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if(rc) return tc_fail("copy kernel failed");
+  *gbps = 2.0 * (double)nbytes * iters / (ms * 1e-3) / 1e9;
   return 0;
 }

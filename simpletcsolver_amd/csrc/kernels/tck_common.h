@@ -14,7 +14,7 @@
 #define TCK_BLOCK 256
 
 __device__ __forceinline__ long long tck_idx(const tc_geom_t & g, int i, int j, int k){
-  return (long long)i + (long long)g.px * ((long long)j + (long long)g.py * (long long)k);
+  return (long long)(i + TC_XOFF) + (long long)g.px * ((long long)j + (long long)g.py * (long long)k);
 }
 
 /* Store an interior value (1 <= j <= ny, 1 <= k <= nz) together with its periodic halo

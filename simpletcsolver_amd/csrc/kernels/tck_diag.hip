@@ -244,17 +244,27 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_diag_walls(const WallArgs A){
   }
 }
 
-/* fixed-order sum of nblocks partial rows of width nq; column 0 is a max when first_is_max */
-__global__ void tck_diag_finish(const double * part, long long nblocks, int nq, int first_is_max, double * out){
-  const int n = threadIdx.x;
-  if(n >= nq) return;
-  double s = (first_is_max && n == 0) ? 0. : 0.;
-  for(long long b = 0; b < nblocks; b++){
-    const double v = part[b * nq + n];
-    if(first_is_max && n == 0) s = fmax(s, v);
-    else s += v;
+/* fixed-order reduction of nblocks partial rows of width nq (column 0 a max when
+ * first_is_max): one block of 256 threads, each thread a strided share, then a tree in LDS;
+ * the order depends only on nblocks, so repeated calls give identical results */
+__global__ void __launch_bounds__(256) tck_diag_finish(const double * part, long long nblocks, int nq, int first_is_max, double * out){
+  __shared__ double red[256];
+  for(int n = 0; n < nq; n++){
+    const bool mx = first_is_max && n == 0;
+    double s = 0.;
+    for(long long b = threadIdx.x; b < nblocks; b += blockDim.x){
+      const double v = part[b * nq + n];
+      s = mx ? fmax(s, v) : s + v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for(int h = 128; h > 0; h >>= 1){
+      if((int)threadIdx.x < h) red[threadIdx.x] = mx ? fmax(red[threadIdx.x], red[threadIdx.x + h]) : red[threadIdx.x] + red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if(threadIdx.x == 0) out[n] = red[0];
+    __syncthreads();
   }
-  out[n] = s;
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -285,7 +295,7 @@ int tck_launch_diag_cells(const tc_geom_t * g, const tc_coef_t * c, const double
   A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.uz = uz; A.t = t; A.part = part; A.scalar = scalar;
   hipLaunchKernelGGL(tck_diag_cells, dim3((g->nx + 1 + 63) / 64, (g->ny + 3) / 4, g->nz), dim3(64, 4), 0, stream, A);
   TCK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(tck_diag_finish, dim3(1), dim3(64), 0, stream, part, tck_diag_cells_blocks(g), NQ_CELL, 1, out);
+  hipLaunchKernelGGL(tck_diag_finish, dim3(1), dim3(256), 0, stream, part, tck_diag_cells_blocks(g), NQ_CELL, 1, out);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -296,7 +306,7 @@ int tck_launch_diag_walls(const tc_geom_t * g, const tc_coef_t * c, const double
   A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.t = t; A.part = part; A.scalar = scalar;
   hipLaunchKernelGGL(tck_diag_walls, dim3((g->ny + TCK_BLOCK - 1) / TCK_BLOCK, g->nz), dim3(TCK_BLOCK), 0, stream, A);
   TCK_CHECK_LAUNCH();
-  hipLaunchKernelGGL(tck_diag_finish, dim3(1), dim3(64), 0, stream, part, tck_diag_walls_blocks(g), 2, 0, out);
+  hipLaunchKernelGGL(tck_diag_finish, dim3(1), dim3(256), 0, stream, part, tck_diag_walls_blocks(g), 2, 0, out);
   TCK_CHECK_LAUNCH();
   return 0;
 }

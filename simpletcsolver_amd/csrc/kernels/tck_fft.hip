@@ -24,7 +24,8 @@
 #include "tck_common.h"
 
 #define FFT_MAXST 16
-#define FFT_EPT 16   /* complex elements per thread per pass (block of 256 -> 4096 per block) */
+#define FFT_EPT 16   /* complex elements per thread per pass */
+#define FFT_MAXTHREADS 1024
 
 struct FftPlan {
   int M;
@@ -165,7 +166,7 @@ struct YFwdArgs {
   const double2 * twN;           /* exp(-2 pi i k / ny), k = 0..M */
 };
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_div_yfft(const YFwdArgs A){
+__global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A){
   extern __shared__ double2 sm[];
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
@@ -224,7 +225,7 @@ struct ZArgs {
   double sign;
 };
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_zfft(const ZArgs A){
+__global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   extern __shared__ double2 sm[];
   const int tz = A.tz;
   const int nz = A.nz;
@@ -269,7 +270,7 @@ __device__ __forceinline__ double2 c2r_pre(double2 xk, double2 xmk, double2 w){
   return make_double2(a.x - wd.y, a.y + wd.x);
 }
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_yifft_psi(const YInvArgs A){
+__global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A){
   extern __shared__ double2 sm[];
   const tc_geom_t & g = A.g;
   const int ti = A.ti;
@@ -351,11 +352,44 @@ int tck_fft_supported(int n){
   return make_plan(n, &P) == 0;
 }
 
-/* batch size per block so that batch * M <= 256 * FFT_EPT and fits 64 KiB of LDS */
-int tck_fft_batch(int M, int cap){
-  int b = 1;
-  while(b * 2 <= cap && (b * 2) * M <= TCK_BLOCK * FFT_EPT) b *= 2;
+/* Tile of TB transforms of length M per block: TB = 16 (one 128 B line of doubles, or two
+ * lines of complex values, per row access) when the LDS allows it, fewer for very long
+ * transforms; threads = TB * M / FFT_EPT. */
+static size_t g_lds_limit = 0;
+
+static size_t lds_limit(void){
+  if(g_lds_limit == 0){
+    int dev = 0;
+    hipGetDevice(&dev);
+    int v = 0;
+    if(hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0){
+      if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
+    }
+    g_lds_limit = (size_t)v;
+  }
+  return g_lds_limit;
+}
+
+static int fft_tile(int M, int cap, int * threads){
+  int b = 16;
+  while(b > 1 && ((size_t)b * M * sizeof(double2) > lds_limit() || b * M > FFT_EPT * FFT_MAXTHREADS)) b /= 2;
+  while(b * 2 <= cap && b * 2 * M <= FFT_EPT * 64) b *= 2;   /* short transforms: wider tiles */
+  if(b > cap) b = cap < 1 ? 1 : cap;
+  int t = (b * M + FFT_EPT - 1) / FFT_EPT;
+  t = (t + 63) / 64 * 64;
+  if(t < 64) t = 64;
+  *threads = t;
   return b;
+}
+
+int tck_fft_batch(int M, int cap){
+  int t;
+  return fft_tile(M, cap, &t);
+}
+
+static int set_lds(const void * fn, size_t bytes){
+  if(bytes <= 65536) return 0;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess ? 0 : -1;
 }
 
 int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
@@ -365,10 +399,12 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.uz = uz; A.dt = dt; A.rkstep = rkstep;
   A.S = (double2 *)S; A.nky = g->ny / 2 + 1;
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
-  A.ti = tck_fft_batch(A.P.M, 64);
+  int nt;
+  A.ti = fft_tile(A.P.M, 64, &nt);
   A.twM = (const double2 *)twM; A.twN = (const double2 *)twN;
   const size_t lds = sizeof(double2) * (size_t)A.ti * A.P.M;
-  hipLaunchKernelGGL(tck_div_yfft, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(TCK_BLOCK), lds, stream, A);
+  if(set_lds((const void *)tck_div_yfft, lds)) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(tck_div_yfft, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(nt), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -378,9 +414,11 @@ int tck_launch_zfft(int nx, int nky, int nz, double * S, const double * tw, int 
   A.nx = nx; A.nky = nky; A.nz = nz; A.S = (double2 *)S; A.tw = (const double2 *)tw;
   A.sign = sign < 0 ? -1. : 1.;
   if(make_plan(nz, &A.P) != 0) return (int)hipErrorInvalidValue;
-  A.tz = tck_fft_batch(nz, 64);
+  int nt;
+  A.tz = fft_tile(nz, 64, &nt);
   const size_t lds = sizeof(double2) * (size_t)A.tz * nz;
-  hipLaunchKernelGGL(tck_zfft, dim3((nx + A.tz - 1) / A.tz, nky), dim3(TCK_BLOCK), lds, stream, A);
+  if(set_lds((const void *)tck_zfft, lds)) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(tck_zfft, dim3((nx + A.tz - 1) / A.tz, nky), dim3(nt), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -390,10 +428,12 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const double * S, double * psi, co
   YInvArgs A;
   A.g = *g; A.S = (const double2 *)S; A.psi = psi; A.nky = g->ny / 2 + 1;
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
-  A.ti = tck_fft_batch(A.P.M, 64);
+  int nt;
+  A.ti = fft_tile(A.P.M, 64, &nt);
   A.twM = (const double2 *)twM; A.twN = (const double2 *)twN;
   const size_t lds = sizeof(double2) * (size_t)A.ti * A.P.M;
-  hipLaunchKernelGGL(tck_yifft_psi, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(TCK_BLOCK), lds, stream, A);
+  if(set_lds((const void *)tck_yifft_psi, lds)) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(tck_yifft_psi, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(nt), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }

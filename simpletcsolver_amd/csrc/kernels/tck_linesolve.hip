@@ -93,61 +93,72 @@ __device__ void factor_periodic(int n, double tl, double tc, double tu, long lon
   *den = tc + tu * q1[0] + tl * q1[stride * (m - 1)];
 }
 
-/* implicit-diffusion factor tables for one field and one substep.
+/* implicit-diffusion factor tables, one job per (field, substep).
  *   x: one system of size n over i = i0..i0+n-1 with per-row stencils lap{l,c,u}[i]
  *   y: one periodic system per i (stencil lap{l,c,u}[i]) of size ny, table layout [j][tp]
  *   z: one periodic system of size nz (stencil lapz)
  * prefactor = fac * (0.5 * gamma * dt * diffusivity) (ux.c:496-506 and its siblings; fac is
  * 2 for the diagonal stress direction of each velocity and for T in y, t.c:346-349).
- * ymeta/zmeta hold {den, l, u} of the periodic systems ([3][tp] and [3]). */
-struct FactorArgs {
-  const double * dt;
+ * ymeta/zmeta hold {den, l, u} of the periodic systems ([3][tp] and [3]).  All jobs of a
+ * step run in one launch: each system is a short sequential recurrence, so the launch costs
+ * one recurrence latency however many systems it holds. */
+#define FACTOR_MAXJOB 12
+
+struct FactorJob {
   int rkstep;
   double diff;
   double fac[3];
-  int dirs[3];
-  int nx, ny, nz, i0, n;
+  int i0, n;
   const double * lxl; const double * lxc; const double * lxu;
   const double * lyl; const double * lyc; const double * lyu;
-  double lzl, lzc, lzu;
-  int tp;
   double * xr; double * xv; double * xl;
   double * yr; double * yv; double * yq1; double * ymeta;
   double * zr; double * zv; double * zq1; double * zmeta;
 };
 
+struct FactorBatch {
+  const double * dt;
+  int dirs[3];
+  int nx, ny, nz, tp;
+  double lzl, lzc, lzu;
+  int njobs;
+  FactorJob jobs[FACTOR_MAXJOB];
+};
+
 __constant__ double c_ls_rka[3] = {32. / 60., 25. / 60., 45. / 60.};
 __constant__ double c_ls_rkb[3] = { 0. / 60., -17. / 60., -25. / 60.};
 
-__global__ void tck_factor(const FactorArgs A){
+__global__ void tck_factor(const FactorBatch B){
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const double dt = *A.dt;
+  if(blockIdx.y >= (unsigned)B.njobs) return;
+  const FactorJob & A = B.jobs[blockIdx.y];
+  const double dt = *B.dt;
   const double gamma = c_ls_rka[A.rkstep] + c_ls_rkb[A.rkstep];
   const double prefactor = 0.5 * gamma * dt * A.diff;
   /* thread 0: x system; threads 1..nx: y systems for i = t; thread nx+1: z system */
-  if(t == 0 && A.dirs[0]){
+  if(t == 0 && B.dirs[0]){
     const double pf = A.fac[0] * prefactor;
     factor_sys(A.n, A.lxl + A.i0, A.lxc + A.i0, A.lxu + A.i0, pf, 1, A.xr, A.xv, A.xl);
   }
-  if(t >= 1 && t <= A.nx && A.dirs[1]){
+  if(t >= 1 && t <= B.nx && B.dirs[1]){
     const int i = t;
     const double pf = A.fac[1] * prefactor;
     const double tl = - pf * A.lyl[i];
     const double tc = 1. - pf * A.lyc[i];
     const double tu = - pf * A.lyu[i];
     double den;
-    factor_periodic(A.ny, tl, tc, tu, A.tp, A.yr + i, A.yv + i, A.yq1 + i, &den);
+    factor_periodic(B.ny, tl, tc, tu, B.tp, A.yr + i, A.yv + i, A.yq1 + i, &den);
     A.ymeta[i] = den;
-    A.ymeta[A.tp + i] = tl;
-    A.ymeta[2 * A.tp + i] = tu;
+    A.ymeta[B.tp + i] = tl;
+    A.ymeta[2 * B.tp + i] = tu;
   }
-  if(t == A.nx + 1 && A.dirs[2]){
+  if(t == B.nx + 1 && B.dirs[2]){
     const double pf = A.fac[2] * prefactor;
-    const double tl = - pf * A.lzl;
-    const double tc = 1. - pf * A.lzc;
-    const double tu = - pf * A.lzu;
+    const double tl = - pf * B.lzl;
+    const double tc = 1. - pf * B.lzc;
+    const double tu = - pf * B.lzu;
     double den;
-    factor_periodic(A.nz, tl, tc, tu, 1, A.zr, A.zv, A.zq1, &den);
+    factor_periodic(B.nz, tl, tc, tu, 1, A.zr, A.zv, A.zq1, &den);
     A.zmeta[0] = den;
     A.zmeta[1] = tl;
     A.zmeta[2] = tu;
@@ -273,35 +284,85 @@ struct XSolveArgs {
   int i0, n;
 };
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
+/* one x line per wavefront; the line is staged through LDS so that global loads and stores
+ * are contiguous across the wave (lane t touches elements t, t+64, ...), while each lane
+ * solves its contiguous segment from LDS.  Segment stride m+1 avoids bank aliasing. */
+__device__ __forceinline__ void xsolve_body(const XSolveArgs & A, double * __restrict__ dd,
+    const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
+    const double * __restrict__ lt, double * sm){
   const tc_geom_t & g = A.g;
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / TCK_WAVE;
+  const int w = threadIdx.x / TCK_WAVE;
+  const int wave = blockIdx.x * (TCK_BLOCK / TCK_WAVE) + w;
   const int lane = threadIdx.x % TCK_WAVE;
   const int nlines = g.ny * g.nz;
-  if(wave >= nlines) return;
-  const int j = 1 + wave % g.ny;
-  const int k = 1 + wave / g.ny;
-  const int seg = (A.n + TCK_WAVE - 1) / TCK_WAVE;
+  const bool live = wave < nlines;
+  const int j = live ? 1 + wave % g.ny : 1;
+  const int k = live ? 1 + wave / g.ny : 1;
+  const int n = A.n;
+  const int seg = (n + TCK_WAVE - 1) / TCK_WAVE;
+  const int ld = seg + 1;
   const int row0 = lane * seg;
-  int cnt = A.n - row0;
+  int cnt = n - row0;
   cnt = cnt < 0 ? 0 : (cnt > seg ? seg : cnt);
   const long long base = tck_idx(g, A.i0, j, k);
+  if(live){
+    for(int idx = lane; idx < n; idx += TCK_WAVE){
+      const int ln = idx / seg;
+      sm[ln * ld + (idx - ln * seg)] = dd[base + idx];
+    }
+  }
+  __syncthreads();
   double q[XS_MAXSEG][1];
   #pragma unroll
-  for(int e = 0; e < XS_MAXSEG; e++) q[e][0] = e < cnt ? A.d[base + row0 + e] : 0.;
-  wave_tdm<1>(q, A.r, A.v, A.l, row0, cnt, A.n);
+  for(int e = 0; e < XS_MAXSEG; e++) q[e][0] = e < cnt ? sm[lane * ld + e] : 0.;
+  wave_tdm<1>(q, rt, vt, lt, row0, cnt, n);
   #pragma unroll
   for(int e = 0; e < XS_MAXSEG; e++){
-    if(e < cnt){
-      const int i = A.i0 + row0 + e;
-      if(A.ucur) tck_store_halo(A.d, g, i, j, k, A.ucur[base + row0 + e] + q[e][0], false);
-      else A.d[base + row0 + e] = q[e][0];
+    if(e < cnt) sm[lane * ld + e] = q[e][0];
+  }
+  __syncthreads();
+  if(!live) return;
+  if(ucur){
+    double u[XS_MAXSEG];
+    #pragma unroll
+    for(int e = 0; e < XS_MAXSEG; e++){
+      const int idx = lane + e * TCK_WAVE;
+      u[e] = idx < n ? ucur[base + idx] : 0.;
+    }
+    #pragma unroll
+    for(int e = 0; e < XS_MAXSEG; e++){
+      const int idx = lane + e * TCK_WAVE;
+      if(idx < n){
+        const int ln = idx / seg;
+        tck_store_halo(dd, g, A.i0 + idx, j, k, u[e] + sm[ln * ld + (idx - ln * seg)], false);
+      }
+    }
+  }else{
+    for(int idx = lane; idx < n; idx += TCK_WAVE){
+      const int ln = idx / seg;
+      dd[base + idx] = sm[ln * ld + (idx - ln * seg)];
     }
   }
 }
 
-/* periodic lines in y (solve_in_y, :39-70) or z (solve_in_z, :72-103): one lane per line,
- * the factors are a table over the line index (and over i for y: layout [row][tp]) */
+/* one x line per wavefront; the line is staged through LDS so that global loads and stores
+ * are contiguous across the wave (lane t touches elements t, t+64, ...), while each lane
+ * solves its contiguous segment from LDS.  Segment stride m+1 avoids bank aliasing. */
+__global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
+  __shared__ double xs_sm[TCK_BLOCK / TCK_WAVE][TCK_WAVE * (XS_MAXSEG + 1)];
+  xsolve_body(A, A.d, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE]);
+}
+
+/* periodic lines in y (solve_in_y, :39-70) or z (solve_in_z, :72-103).  A block holds TI
+ * lines (one per lane column, consecutive i, so each row access is a TI * 8 B contiguous
+ * chunk) split into S segments along the line; every thread keeps its segment in registers,
+ * so each element is read once and written once.  The forward and backward substitutions of
+ * the (n-1) leading block are affine recurrences: each segment composes its rows, a fold over
+ * the earlier (later) segments in LDS gives the value entering it, and the segment is replayed
+ * with the reference's update formulas (tdm.c:17-50).  The Sherman-Morrison step
+ * (tdm.c:92-98) then needs q0[0], q0[n-2] and q[n-1], shared through LDS. */
+#define PS_MAXM 32
+
 struct PSolveArgs {
   tc_geom_t g;
   double * d;
@@ -314,55 +375,144 @@ struct PSolveArgs {
   int i0, isz;
 };
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_psolve(const PSolveArgs A){
+/* body with restrict-qualified pointers: loads of the old field and the tables may be
+ * scheduled ahead of the stores to the increment array */
+__device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __restrict__ d,
+    const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
+    const double * __restrict__ q1t, const double * __restrict__ meta, double * ps_sm){
   const tc_geom_t & g = A.g;
-  const int i = A.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int TI = blockDim.x;
+  const int S = blockDim.y;
+  const int ti = threadIdx.x;
+  const int sg = threadIdx.y;
+  /* tiles start at i = 1 (a 128 B line boundary); rows below i0 are skipped (ux: i0 = 2) */
+  const int i = 1 + blockIdx.x * TI + ti;
   const int other = 1 + blockIdx.y;    /* k for y lines, j for z lines */
-  if(i >= A.i0 + A.isz) return;
+  const bool valid = i >= A.i0 && i < A.i0 + A.isz;
   const int n = A.dir == 1 ? g.ny : g.nz;
   const int m = n - 1;
+  const int seg = (m + S - 1) / S;
+  const int row0 = sg * seg;
+  int cnt = m - row0;
+  cnt = cnt < 0 ? 0 : (cnt > seg ? seg : cnt);
   const long long step = A.dir == 1 ? (long long)g.px : g.plane;
-  const long long base = A.dir == 1 ? tck_idx(g, i, 1, other) : tck_idx(g, i, other, 1);
-  const double * r = A.r + (A.tcol ? i : 0);
-  const double * v = A.v + (A.tcol ? i : 0);
-  const double * q1 = A.q1 + (A.tcol ? i : 0);
+  const int ic = valid ? i : A.i0;
+  const long long base = A.dir == 1 ? tck_idx(g, ic, 1, other) : tck_idx(g, ic, other, 1);
+  const int col = A.tcol ? ic : 0;
   const long long ts = A.tstride;
-  const int col = A.tcol ? i : 0;
-  const double den = A.meta[col];
-  const double tl = A.meta[ts + col];
-  const double tu = A.meta[2 * ts + col];
-  double * d = A.d;
-  /* gtsv on the leading (n-1) block (tdm.c:86-90) */
-  double qprev = d[base] / r[0];
-  d[base] = qprev;
-  for(int s = 1; s < m; s++){
-    const double q = r[ts * s] * (d[base + step * s] - tl * qprev);
-    d[base + step * s] = q;
-    qprev = q;
-  }
-  const double q_m1 = qprev;   /* q0[n-2] */
-  double qn = qprev;
-  for(int s = m - 2; s >= 0; s--){
-    const double q = d[base + step * s] - v[ts * s] * qn;
-    d[base + step * s] = q;
-    qn = q;
-  }
-  const double q_0 = qn;       /* q0[0] */
-  /* x_{n-1} (tdm.c:92-94) */
-  const double num = d[base + step * m] - tu * q_0 - tl * q_m1;
-  const double xl = fabs(den) < DBL_EPSILON ? 0. : num / den;
-  /* x_i += x_{n-1} q1_i (tdm.c:96-98) and the field update */
-  for(int s = 0; s < n; s++){
-    const long long o = base + step * s;
-    const double x = s < m ? d[o] + xl * q1[ts * s] : xl;
-    if(A.ucur){
-      const int jj = A.dir == 1 ? 1 + s : other;
-      const int kk = A.dir == 1 ? other : 1 + s;
-      tck_store_halo(d, g, i, jj, kk, A.ucur[o] + x, false);
-    }else{
-      d[o] = x;
+  const double * r = rt + col;
+  const double * v = vt + col;
+  const double * q1 = q1t + col;
+  const double den = meta[col];
+  const double tl = meta[ts + col];
+  const double tu = meta[2 * ts + col];
+  double * smA = ps_sm;                  /* [S][TI] */
+  double * smB = ps_sm + S * TI;         /* [S][TI] */
+  double * smX = ps_sm + 2 * S * TI;     /* [3][TI]: q0[0], q0[n-2], q[n-1] */
+  double q[PS_MAXM];
+  #pragma unroll
+  for(int e = 0; e < PS_MAXM; e++) q[e] = (valid && e < cnt) ? d[base + step * (row0 + e)] : 0.;
+  if(valid && sg == S - 1) smX[2 * TI + ti] = d[base + step * m];
+  /* forward: q0 = q0 / c0, q_i = r_i (q_i - l q_{i-1}) */
+  double a = 1., b = 0.;
+  #pragma unroll
+  for(int e = 0; e < PS_MAXM; e++){
+    if(e < cnt){
+      const int row = row0 + e;
+      if(row == 0){ a = 0.; b = q[e] / r[0]; }
+      else {
+        const double ri = r[ts * row];
+        const double ai = - ri * tl;
+        b = ai * b + ri * q[e];
+        a = ai * a;
+      }
     }
   }
+  smA[sg * TI + ti] = a;
+  smB[sg * TI + ti] = b;
+  __syncthreads();
+  double in = 0.;
+  for(int s2 = 0; s2 < sg; s2++) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
+  #pragma unroll
+  for(int e = 0; e < PS_MAXM; e++){
+    if(e < cnt){
+      const int row = row0 + e;
+      if(row == 0) q[e] = q[e] / r[0];
+      else q[e] = r[ts * row] * (q[e] - tl * in);
+      in = q[e];
+      if(row == m - 1) smX[TI + ti] = q[e];
+    }
+  }
+  __syncthreads();
+  /* backward: q_i -= v_i q_{i+1}, q0[n-2] kept */
+  a = 1.;
+  b = 0.;
+  #pragma unroll
+  for(int e = PS_MAXM - 1; e >= 0; e--){
+    if(e < cnt){
+      const int row = row0 + e;
+      if(row == m - 1){ a = 0.; b = q[e]; }
+      else {
+        const double ai = - v[ts * row];
+        b = ai * b + q[e];
+        a = ai * a;
+      }
+    }
+  }
+  smA[sg * TI + ti] = a;
+  smB[sg * TI + ti] = b;
+  __syncthreads();
+  in = 0.;
+  for(int s2 = S - 1; s2 > sg; s2--) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
+  #pragma unroll
+  for(int e = PS_MAXM - 1; e >= 0; e--){
+    if(e < cnt){
+      const int row = row0 + e;
+      if(row != m - 1) q[e] -= v[ts * row] * in;
+      in = q[e];
+    }
+  }
+  if(cnt > 0 && row0 == 0) smX[ti] = q[0];
+  __syncthreads();
+  /* x_{n-1} (tdm.c:92-94) and x_i += x_{n-1} q1_i (:96-98) */
+  const double num = smX[2 * TI + ti] - tu * smX[ti] - tl * smX[TI + ti];
+  const double xl = fabs(den) < DBL_EPSILON ? 0. : num / den;
+  if(!valid) return;
+  if(ucur){
+    double u[PS_MAXM];
+    #pragma unroll
+    for(int e = 0; e < PS_MAXM; e++) u[e] = e < cnt ? ucur[base + step * (row0 + e)] : 0.;
+    const double ul = sg == S - 1 ? ucur[base + step * m] : 0.;
+    #pragma unroll
+    for(int e = 0; e < PS_MAXM; e++){
+      if(e < cnt){
+        const int row = row0 + e;
+        const double x = q[e] + xl * q1[ts * row];
+        const int jj = A.dir == 1 ? 1 + row : other;
+        const int kk = A.dir == 1 ? other : 1 + row;
+        tck_store_halo(d, g, i, jj, kk, u[e] + x, false);
+      }
+    }
+    if(sg == S - 1){
+      const int jj = A.dir == 1 ? n : other;
+      const int kk = A.dir == 1 ? other : n;
+      tck_store_halo(d, g, i, jj, kk, ul + xl, false);
+    }
+  }else{
+    #pragma unroll
+    for(int e = 0; e < PS_MAXM; e++){
+      if(e < cnt){
+        const int row = row0 + e;
+        d[base + step * row] = q[e] + xl * q1[ts * row];
+      }
+    }
+    if(sg == S - 1) d[base + step * m] = xl;
+  }
+}
+
+__global__ void __launch_bounds__(1024) tck_psolve(const PSolveArgs A){
+  extern __shared__ double ps_sm[];
+  psolve_body(A, A.d, A.ucur, A.r, A.v, A.q1, A.meta, ps_sm);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -416,23 +566,37 @@ struct PXSolveArgs {
   const double * pr; const double * pv; const double * pl;
 };
 
+/* one (ky, kz) line per wavefront, staged through LDS as in tck_xsolve so that the spectrum
+ * row (nx complex values, contiguous) is read and written with contiguous wave accesses */
 __global__ void __launch_bounds__(TCK_BLOCK) tck_pxsolve(const PXSolveArgs A){
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / TCK_WAVE;
+  extern __shared__ double2 px_sm[];
+  const int w = threadIdx.x / TCK_WAVE;
+  const long long wave = (long long)blockIdx.x * (blockDim.x / TCK_WAVE) + w;
   const int lane = threadIdx.x % TCK_WAVE;
-  if(wave >= A.nlines) return;
+  const bool live = wave < A.nlines;
   const int n = A.nx;
   const int seg = (n + TCK_WAVE - 1) / TCK_WAVE;
+  const int ld = seg + 1;
   const int row0 = lane * seg;
   int cnt = n - row0;
   cnt = cnt < 0 ? 0 : (cnt > seg ? seg : cnt);
-  double2 * row = A.s + wave * n;
-  const double * r = A.pr + wave * n;
-  const double * v = A.pv + wave * n;
+  const long long lw = live ? wave : 0;
+  double2 * row = A.s + lw * n;
+  const double * r = A.pr + lw * n;
+  const double * v = A.pv + lw * n;
+  double2 * sm = px_sm + (size_t)w * TCK_WAVE * ld;
+  if(live){
+    for(int idx = lane; idx < n; idx += TCK_WAVE){
+      const int ln = idx / seg;
+      sm[ln * ld + (idx - ln * seg)] = row[idx];
+    }
+  }
+  __syncthreads();
   double q[XS_MAXSEG][2];
   #pragma unroll
   for(int e = 0; e < XS_MAXSEG; e++){
     if(e < cnt){
-      const double2 x = row[row0 + e];
+      const double2 x = sm[lane * ld + e];
       q[e][0] = x.x;
       q[e][1] = x.y;
     }else{
@@ -442,32 +606,45 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_pxsolve(const PXSolveArgs A){
   wave_tdm<2>(q, r, v, A.pl, row0, cnt, n);
   #pragma unroll
   for(int e = 0; e < XS_MAXSEG; e++){
-    if(e < cnt) row[row0 + e] = make_double2(q[e][0], q[e][1]);
+    if(e < cnt) sm[lane * ld + e] = make_double2(q[e][0], q[e][1]);
+  }
+  __syncthreads();
+  if(!live) return;
+  for(int idx = lane; idx < n; idx += TCK_WAVE){
+    const int ln = idx / seg;
+    row[idx] = sm[ln * ld + (idx - ln * seg)];
   }
 }
 
 /* ------------------------------------------------------------------------------------- */
 extern "C" {
 
-int tck_launch_factor(const double * dt, int rkstep, double diff, const double fac[3],
-    const int dirs[3], int nx, int ny, int nz, int i0, int n,
-    const double * lxl, const double * lxc, const double * lxu,
-    const double * lyl, const double * lyc, const double * lyu,
-    double lzl, double lzc, double lzu, int tp,
-    double * xr, double * xv, double * xl,
-    double * yr, double * yv, double * yq1, double * ymeta,
-    double * zr, double * zv, double * zq1, double * zmeta, hipStream_t stream){
-  FactorArgs A;
-  A.dt = dt; A.rkstep = rkstep; A.diff = diff;
-  for(int d = 0; d < 3; d++){ A.fac[d] = fac[d]; A.dirs[d] = dirs[d]; }
-  A.nx = nx; A.ny = ny; A.nz = nz; A.i0 = i0; A.n = n;
-  A.lxl = lxl; A.lxc = lxc; A.lxu = lxu; A.lyl = lyl; A.lyc = lyc; A.lyu = lyu;
-  A.lzl = lzl; A.lzc = lzc; A.lzu = lzu; A.tp = tp;
-  A.xr = xr; A.xv = xv; A.xl = xl;
-  A.yr = yr; A.yv = yv; A.yq1 = yq1; A.ymeta = ymeta;
-  A.zr = zr; A.zv = zv; A.zq1 = zq1; A.zmeta = zmeta;
+/* jobs: njobs descriptors laid out as in FactorJob; fields beyond the implicit set may be null */
+int tck_launch_factor(const double * dt, const int dirs[3], int nx, int ny, int nz, int tp,
+    double lzl, double lzc, double lzu, int njobs, const int * rkstep, const double * diff,
+    const double (*fac)[3], const int * i0, const int * n,
+    const double * const (*lx)[3], const double * const (*ly)[3],
+    double * const (*xt)[3], double * const (*yt)[4], double * const (*zt)[4], hipStream_t stream){
+  if(njobs < 1 || njobs > FACTOR_MAXJOB) return (int)hipErrorInvalidValue;
+  FactorBatch B;
+  B.dt = dt;
+  for(int d = 0; d < 3; d++) B.dirs[d] = dirs[d];
+  B.nx = nx; B.ny = ny; B.nz = nz; B.tp = tp;
+  B.lzl = lzl; B.lzc = lzc; B.lzu = lzu;
+  B.njobs = njobs;
+  for(int q = 0; q < njobs; q++){
+    FactorJob & J = B.jobs[q];
+    J.rkstep = rkstep[q]; J.diff = diff[q];
+    for(int d = 0; d < 3; d++) J.fac[d] = fac[q][d];
+    J.i0 = i0[q]; J.n = n[q];
+    J.lxl = lx[q][0]; J.lxc = lx[q][1]; J.lxu = lx[q][2];
+    J.lyl = ly[q][0]; J.lyc = ly[q][1]; J.lyu = ly[q][2];
+    J.xr = xt[q][0]; J.xv = xt[q][1]; J.xl = xt[q][2];
+    J.yr = yt[q][0]; J.yv = yt[q][1]; J.yq1 = yt[q][2]; J.ymeta = yt[q][3];
+    J.zr = zt[q][0]; J.zv = zt[q][1]; J.zq1 = zt[q][2]; J.zmeta = zt[q][3];
+  }
   const int nthreads = nx + 2;
-  hipLaunchKernelGGL(tck_factor, dim3((nthreads + 63) / 64), dim3(64), 0, stream, A);
+  hipLaunchKernelGGL(tck_factor, dim3((nthreads + 63) / 64, njobs), dim3(64), 0, stream, B);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -491,8 +668,16 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, double * d, const double * u
   PSolveArgs A;
   A.g = *g; A.d = d; A.ucur = ucur; A.r = r; A.v = v; A.q1 = q1; A.meta = meta;
   A.tstride = tstride; A.tcol = tcol; A.dir = dir; A.i0 = i0; A.isz = isz;
+  const int n = dir == 1 ? g->ny : g->nz;
+  const int m = n - 1;
+  const int S = (m + PS_MAXM - 1) / PS_MAXM;
+  if(S > 64) return (int)hipErrorInvalidValue;
+  int TI = 512 / S;
+  TI = TI > 64 ? 64 : (TI < 16 ? 16 : TI);
   const int nother = dir == 1 ? g->nz : g->ny;
-  hipLaunchKernelGGL(tck_psolve, dim3((isz + 63) / 64, nother), dim3(64), 0, stream, A);
+  const size_t lds = sizeof(double) * (size_t)(2 * S * TI + 3 * TI);
+  const int span = i0 + isz - 1;   /* tiles cover i = 1 .. i0+isz-1 */
+  hipLaunchKernelGGL(tck_psolve, dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -514,9 +699,13 @@ int tck_launch_pxsolve(int nx, int nlines, double * s, const double * pr, const 
   if(nx > TCK_WAVE * XS_MAXSEG || nx < 2) return (int)hipErrorInvalidValue;
   PXSolveArgs A;
   A.nx = nx; A.nlines = nlines; A.s = (double2 *)s; A.pr = pr; A.pv = pv; A.pl = pl;
+  const int seg = (nx + TCK_WAVE - 1) / TCK_WAVE;
   const int waves_per_block = TCK_BLOCK / TCK_WAVE;
+  const size_t lds = sizeof(double2) * (size_t)waves_per_block * TCK_WAVE * (seg + 1);
   const long long nblocks = ((long long)nlines + waves_per_block - 1) / waves_per_block;
-  hipLaunchKernelGGL(tck_pxsolve, dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
+  if(lds > 65536 && hipFuncSetAttribute((const void *)tck_pxsolve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(tck_pxsolve, dim3((unsigned)nblocks), dim3(TCK_BLOCK), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }

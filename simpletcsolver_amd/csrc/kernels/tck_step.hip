@@ -36,7 +36,15 @@ struct PredictArgs {
   int final_here;   /* 1: no line solve follows, write u + delta with halos */
 };
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
+/* body with restrict-qualified pointers, so that the loads of each field section are free
+ * to move ahead of the stores of the previous one */
+__device__ __forceinline__ void predict_body(const PredictArgs & A,
+    const double * __restrict__ ux, const double * __restrict__ uy, const double * __restrict__ uz,
+    const double * __restrict__ t, const double * __restrict__ p,
+    double * __restrict__ nxo, double * __restrict__ nyo, double * __restrict__ nzo, double * __restrict__ nto,
+    const double * __restrict__ sbx, const double * __restrict__ sby, const double * __restrict__ sbz,
+    const double * __restrict__ sbt,
+    double * __restrict__ sax, double * __restrict__ say, double * __restrict__ saz, double * __restrict__ sat){
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -46,10 +54,6 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
   const long long o = tck_idx(g, i, j, k);
   const long long sy = g.px;
   const long long sz = g.plane;
-  const double * __restrict__ ux = A.ux;
-  const double * __restrict__ uy = A.uy;
-  const double * __restrict__ uz = A.uz;
-  const double * __restrict__ p = A.p;
   const double nu = c.nu;
   const double nu2 = 2. * nu;
   const double dt = *A.dt;
@@ -128,11 +132,11 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
     /* pressure :377-394 */
     gg -= c.ihxf[i] * (- p[o - 1] + p[o]);
     double b = 0.;
-    if(A.sbx) b = A.sbx[o];
-    const double delta = (A.sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(A.sax) A.sax[o] = a;
-    if(A.final_here) tck_store_halo(A.nx_, g, i, j, k, u0 + delta, false);
-    else A.nx_[o] = delta;
+    if(sbx) b = sbx[o];
+    const double delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
+    if(sax) sax[o] = a;
+    if(A.final_here) tck_store_halo(nxo, g, i, j, k, u0 + delta, false);
+    else nxo[o] = delta;
   }
 
   /* ---------------- uy, centres i = 1..nx (predict/uy.c) ---------------- */
@@ -214,11 +218,11 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
     /* pressure :402-419 */
     gg -= c.ihyc[i] * (- p[o - sy] + p[o]);
     double b = 0.;
-    if(A.sby) b = A.sby[o];
-    const double delta = (A.sby ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(A.say) A.say[o] = a;
-    if(A.final_here) tck_store_halo(A.ny_, g, i, j, k, v0 + delta, false);
-    else A.ny_[o] = delta;
+    if(sby) b = sby[o];
+    const double delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
+    if(say) say[o] = a;
+    if(A.final_here) tck_store_halo(nyo, g, i, j, k, v0 + delta, false);
+    else nyo[o] = delta;
   }
 
   /* ---------------- uz, centres (predict/uz.c) ---------------- */
@@ -280,16 +284,15 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
     /* pressure :301-318 */
     gg -= c.ihz * (- p[o - sz] + p[o]);
     double b = 0.;
-    if(A.sbz) b = A.sbz[o];
-    const double delta = (A.sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(A.saz) A.saz[o] = a;
-    if(A.final_here) tck_store_halo(A.nz_, g, i, j, k, w0 + delta, false);
-    else A.nz_[o] = delta;
+    if(sbz) b = sbz[o];
+    const double delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
+    if(saz) saz[o] = a;
+    if(A.final_here) tck_store_halo(nzo, g, i, j, k, w0 + delta, false);
+    else nzo[o] = delta;
   }
 
   /* ---------------- T, centres (predict/t.c) ---------------- */
   if(A.f.scalar){
-    const double * __restrict__ t = A.t;
     const double kappa = c.kappa;
     double a = 0., gg = 0.;
     const double t0 = t[o], txm = t[o - 1], txp = t[o + 1];
@@ -328,12 +331,17 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
       if(iz) gg += vz; else a += vz;
     }
     double b = 0.;
-    if(A.sbt) b = A.sbt[o];
-    const double delta = (A.sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(A.sat) A.sat[o] = a;
-    if(A.final_here) tck_store_halo(A.nt, g, i, j, k, t0 + delta, false);
-    else A.nt[o] = delta;
+    if(sbt) b = sbt[o];
+    const double delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
+    if(sat) sat[o] = a;
+    if(A.final_here) tck_store_halo(nto, g, i, j, k, t0 + delta, false);
+    else nto[o] = delta;
   }
+}
+
+__global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
+  predict_body(A, A.ux, A.uy, A.uz, A.t, A.p, A.nx_, A.ny_, A.nz_, A.nt, A.sbx, A.sby, A.sbz, A.sbt,
+      A.sax, A.say, A.saz, A.sat);
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -350,7 +358,9 @@ struct CorrectArgs {
   unsigned long long * dtmin;   /* non-null: CFL minimum of the corrected field */
 };
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
+__device__ __forceinline__ void correct_body(const CorrectArgs & A, double * __restrict__ ux,
+    double * __restrict__ uy, double * __restrict__ uz, double * __restrict__ pp,
+    const double * __restrict__ psi){
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -362,7 +372,6 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
     const long long o = tck_idx(g, i, j, k);
     const long long sy = g.px;
     const long long sz = g.plane;
-    const double * __restrict__ psi = A.psi;
     const double dt = *A.dt;
     const int s = A.rkstep;
     const double gamma = c_rka[s] + c_rkb[s];
@@ -372,24 +381,24 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
     if(A.mode & 1){
     if(i >= 2){ /* correct/ux.c:25-52 */
       const double hx = c.hxxf[i];
-      const double v = A.ux[o] - prefactor / hx * (- psi[o - 1] + ps0);
-      tck_store_halo(A.ux, g, i, j, k, v, false);
+      const double v = ux[o] - prefactor / hx * (- psi[o - 1] + ps0);
+      tck_store_halo(ux, g, i, j, k, v, false);
       lmin = fmin(lmin, hx / (fabs(v) + small));
     }
     { /* correct/uy.c:25-52 */
       const double hy = c.hyxc[i];
-      const double v = A.uy[o] - prefactor / hy * (- psi[o - sy] + ps0);
-      tck_store_halo(A.uy, g, i, j, k, v, false);
+      const double v = uy[o] - prefactor / hy * (- psi[o - sy] + ps0);
+      tck_store_halo(uy, g, i, j, k, v, false);
       lmin = fmin(lmin, hy / (fabs(v) + small));
     }
     { /* correct/uz.c:15-44 */
-      const double v = A.uz[o] - prefactor / c.hz * (- psi[o - sz] + ps0);
-      tck_store_halo(A.uz, g, i, j, k, v, false);
+      const double v = uz[o] - prefactor / c.hz * (- psi[o - sz] + ps0);
+      tck_store_halo(uz, g, i, j, k, v, false);
       lmin = fmin(lmin, c.hz / (fabs(v) + small));
     }
     }
     if(A.mode & 2){ /* update_pressure.c:146-173 */
-      double v = A.p[o] + ps0;
+      double v = pp[o] + ps0;
       const double pf = 0.5 * gamma * dt * c.nu;
       if(A.f.implicit[0]){
         const double l = c.plx_l[i];
@@ -409,7 +418,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
         const double cc = - l - u;
         v -= 2. * pf / c.jdxc[i] * (+ l * psi[o - sz] + cc * ps0 + u * psi[o + sz]);
       }
-      tck_store_halo(A.p, g, i, j, k, v, true);
+      tck_store_halo(pp, g, i, j, k, v, true);
     }
   }
   if(A.dtmin){
@@ -425,6 +434,10 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
       atomicMin(A.dtmin, tck_bits(m));
     }
   }
+}
+
+__global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
+  correct_body(A, A.ux, A.uy, A.uz, A.p, A.psi);
 }
 
 /* decide_dt_adv on the current field (decide_dt.c:32-80), for the first step and after

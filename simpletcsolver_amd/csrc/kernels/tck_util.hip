@@ -1,0 +1,78 @@
+/*
+ * tck_util.hip -- device initial-condition fill and the bandwidth probe (gfx950).
+ *
+ *  tck_ic_fill   seeded field of the IC tool (host/tc_ic.c states the definition): per-i
+ *                profile + amp (2u - 1) splitmix64 noise + mode_amp X(i) Y(j) Z(k), evaluated
+ *                on the device with the same integer hash and the same operand order as
+ *                the host generator, so both give identical values.
+ *  tck_copy      dst = src over n doubles, 16 B per lane per access: the measured HBM ceiling
+ *                and the reference kernel for scaling hardware byte counters.
+ */
+#include "tck_common.h"
+
+__device__ __forceinline__ unsigned long long tck_splitmix64(unsigned long long z){
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+struct IcArgs {
+  tc_geom_t g;
+  double * dst;
+  const double * prof;    /* per NPY x index q */
+  const double * mx;      /* per q, or null */
+  const double * my;      /* per j */
+  const double * mz;      /* per k */
+  double amp, mode_amp;
+  unsigned long long seed;
+  unsigned field;
+  int nxs, i0;
+};
+
+__global__ void tck_ic_fill(const IcArgs A){
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;        /* 0..ny-1 */
+  const int k = blockIdx.z;        /* 0..nz-1 */
+  if(q >= A.nxs) return;
+  double v = A.prof[q];
+  if(A.amp != 0.){
+    const unsigned long long lin = ((unsigned long long)k * A.g.ny + j) * A.nxs + q;
+    const unsigned long long key = A.seed * 0x9E3779B97F4A7C15ULL + ((unsigned long long)A.field << 56) + lin;
+    const double u = (double)(tck_splitmix64(key) >> 11) * 0x1.0p-53;
+    v = v + A.amp * (2. * u - 1.);
+  }
+  if(A.mx) v = v + A.mode_amp * A.mx[q] * A.my[j] * A.mz[k];
+  A.dst[tck_idx(A.g, A.i0 + q, j + 1, k + 1)] = v;
+}
+
+__global__ void tck_copy(const double2 * __restrict__ src, double2 * __restrict__ dst, long long n2){
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for(long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += stride){
+    dst[t] = src[t];
+  }
+}
+
+extern "C" {
+
+int tck_launch_ic_fill(const tc_geom_t * g, double * dst, const double * prof, const double * mx,
+    const double * my, const double * mz, double amp, double mode_amp, unsigned long long seed,
+    unsigned field, int nxs, int i0, hipStream_t stream){
+  IcArgs A;
+  A.g = *g; A.dst = dst; A.prof = prof; A.mx = mx; A.my = my; A.mz = mz; A.amp = amp;
+  A.mode_amp = mode_amp; A.seed = seed; A.field = field; A.nxs = nxs; A.i0 = i0;
+  hipLaunchKernelGGL(tck_ic_fill, dim3((nxs + 63) / 64, g->ny, g->nz), dim3(64), 0, stream, A);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
+int tck_launch_copy(const double * src, double * dst, long long n, hipStream_t stream){
+  const long long n2 = n / 2;
+  const int nblocks = 256 * 8 * 4;
+  hipLaunchKernelGGL(tck_copy, dim3(nblocks), dim3(TCK_BLOCK), 0, stream,
+      (const double2 *)src, (double2 *)dst, n2);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
+}

@@ -2,11 +2,12 @@
  * tc_shared.h -- plain-C structs shared by the C host code and the HIP kernels.
  *
  * Device layout of every 3-D array (velocities, scalar, pressure, potential, RK buffers):
- *   idx(i, j, k) = i + px * (j + py * k),  i = 0..nx+1, j = 0..ny+1, k = 0..nz+1
+ *   idx(i, j, k) = (i + xoff) + px * (j + py * k),  i = 0..nx+1, j = 0..ny+1, k = 0..nz+1
  * with the reference's own indices (src/fluid/*, include/array_macros/fluid/*.h): x walls at
  * i = 0 and nx+1 (ux: faces i = 1..nx+1, slot 0 unused), one periodic halo plane on each
- * side in y and z.  px is nx+2 rounded up to a multiple of 8 doubles (64 B rows).  All
- * arrays share one index map so a kernel addresses ux, uy, p and the RK buffers alike.
+ * side in y and z.  xoff = 15 and px a multiple of 16 doubles put i = 1 on a 128 B cache-line
+ * boundary in every row, so tiles of 16, 32 or 64 interior cells read and write whole lines.
+ * All arrays share one index map so a kernel addresses ux, uy, p and the RK buffers alike.
  */
 #ifndef TC_SHARED_H
 #define TC_SHARED_H
@@ -14,6 +15,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+#define TC_XOFF 15
 
 typedef struct {
   int nx, ny, nz;
