@@ -57,6 +57,12 @@ const char * tc_last_error(void);
  * field's boundary condition and periodic halos (fluid/init.c:112-116). */
 int tc_set_field(tc_solver_t * s, const char * name, const double * data);
 int tc_get_field(tc_solver_t * s, const char * name, double * data);
+/* the same in the reference's host array layout, with the y/z halo planes:
+ * (nz+2, ny+2, nx+1) for ux, (nz+2, ny+2, nx+2) otherwise (src/array.c:10-46) */
+int tc_set_field_ref(tc_solver_t * s, const char * name, const double * data);
+int tc_get_field_ref(tc_solver_t * s, const char * name, double * data);
+/* decide_dt coefficients (decide_dt.c:128-129) and a fixed step (> 0) */
+int tc_set_dt_params(tc_solver_t * s, double coef_dt_adv, double coef_dt_dif, double dt_fixed);
 
 /* one time step, integrate() (integrate.c:9-35); *dt receives the step size used */
 int tc_step(tc_solver_t * s, double * dt);
@@ -83,6 +89,19 @@ int tc_total_energy(tc_solver_t * s, double out[4]);
 /* on: every following step records the injection and dissipation of the field at the start
  * of its last substep (the reference's logging reads the tensors that substep left) */
 int tc_arm_diagnostics(tc_solver_t * s, int on);
+
+/* running statistics, replacing statistics.init/collect/output (src/statistics.c:56-307):
+ * device accumulators in the reference's layouts, ux1 ux2 adv dif with (nx+1) x ny x nz and
+ * uy1 uy2 uz1 uz2 with (nx+2) x ny x nz doubles (i fastest, no halos).  tc_stat_arm(s, 1)
+ * allocates and zeroes them on first use and makes every following step keep the
+ * LYX0+LYX1+LXY tensors of its last substep (statistics.c:221-234 reads those);
+ * tc_stat_collect adds one sample of the current field; tc_stat_get copies one accumulator
+ * ("ux1", "ux2", "uy1", "uy2", "uz1", "uz2", "adv", "dif") to host and/or returns the sample
+ * count */
+int tc_stat_arm(tc_solver_t * s, int on);
+int tc_stat_collect(tc_solver_t * s);
+int tc_stat_reset(tc_solver_t * s);
+int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num);
 
 /* events around the kernels of one named stage ("predict", "poisson", ...) while timing */
 int tc_profile_enable(tc_solver_t * s, int on);

@@ -38,6 +38,9 @@ struct orc_t_ {
   double * eval_y; double * eval_z;     /* :311-349 */
   double * ptdm_l; double * ptdm_u;     /* :173-211 */
   double * spec;                        /* complex nx x (ny/2+1) x nz, i fastest */
+  /* statistics.c accumulators (allocated by the first orc_stat_collect) */
+  double * st[8];
+  size_t st_num;
 };
 
 /* ---- layouts: include/array_macros/fluid/*.h, include/array_macros/domain/*.h ---- */
@@ -518,6 +521,7 @@ void orc_destroy(orc_t * o){
   for(size_t n = 0; n < sizeof(bufs) / sizeof(bufs[0]); n++) free(bufs[n]);
   for(int n = 0; n < 3; n++){ free(o->srcux[n]); free(o->srcuy[n]); free(o->srcuz[n]); free(o->srct[n]); }
   free(o->lapx_ux); free(o->lapy_ux); free(o->lapx_c); free(o->lapy_c);
+  for(int n = 0; n < 8; n++) free(o->st[n]);
   free(o);
 }
 
@@ -541,6 +545,10 @@ double * orc_array(orc_t * o, const char * name, size_t * nitems){
     {"srct0", o->srct[0], nx * ny * nz}, {"srct1", o->srct[1], nx * ny * nz}, {"srct2", o->srct[2], nx * ny * nz},
     {"xf", o->xf, nx + 1}, {"xc", o->xc, nx + 2}, {"hxxf", o->hxxf, nx + 1}, {"hxxc", o->hxxc, nx + 2},
     {"hyxf", o->hyxf, nx + 1}, {"hyxc", o->hyxc, nx + 2}, {"jdxf", o->jdxf, nx + 1}, {"jdxc", o->jdxc, nx + 2},
+    {"ux1", o->st[0], (nx + 1) * ny * nz}, {"ux2", o->st[1], (nx + 1) * ny * nz},
+    {"uy1", o->st[2], (nx + 2) * ny * nz}, {"uy2", o->st[3], (nx + 2) * ny * nz},
+    {"uz1", o->st[4], (nx + 2) * ny * nz}, {"uz2", o->st[5], (nx + 2) * ny * nz},
+    {"adv", o->st[6], (nx + 1) * ny * nz}, {"dif", o->st[7], (nx + 1) * ny * nz},
   };
   for(size_t n = 0; n < sizeof(tab) / sizeof(tab[0]); n++){
     if(0 == strcmp(tab[n].s, name)){
@@ -1524,3 +1532,49 @@ void orc_total_energy(const orc_t * o, double out[4]){
     q[3] += JDXC(i) * 0.5 * pow(T(i, j, k), 2.);
   for(int n = 0; n < 4; n++) out[n] = q[n];
 }
+
+/* ---------------------------------------------------------------------------------------
+ * statistics, src/statistics.c:111-255: one sample of the current field; the dif term reads
+ * the tensor arrays as the last predict left them
+ * ------------------------------------------------------------------------------------- */
+#define SF(i, j, k) ((size_t)((i) - 1) + (size_t)(NX + 1) * ((size_t)((j) - 1) + (size_t)NY * (size_t)((k) - 1)))
+#define SC(i, j, k) ((size_t)(i) + (size_t)(NX + 2) * ((size_t)((j) - 1) + (size_t)NY * (size_t)((k) - 1)))
+
+size_t orc_stat_collect(orc_t * o){
+  if(!o->st[0]){
+    const size_t nf = (size_t)(NX + 1) * NY * NZ, nc = (size_t)(NX + 2) * NY * NZ;
+    const size_t n[8] = {nf, nf, nc, nc, nc, nc, nf, nf};
+    for(int q = 0; q < 8; q++) o->st[q] = calloc(n[q], sizeof(double));
+    o->st_num = 0;
+  }
+  double * ux1 = o->st[0], * ux2 = o->st[1], * uy1 = o->st[2], * uy2 = o->st[3];
+  double * uz1 = o->st[4], * uz2 = o->st[5], * adv = o->st[6], * dif = o->st[7];
+  /* collect_mean_ux, :117-136 */
+  for(int k = 1; k <= NZ; k++) for(int j = 1; j <= NY; j++) for(int i = 1; i <= NX + 1; i++){
+    ux1[SF(i, j, k)] += pow(UX(i, j, k), 1.);
+    ux2[SF(i, j, k)] += pow(UX(i, j, k), 2.);
+  }
+  /* collect_mean_uy, :143-162 */
+  for(int k = 1; k <= NZ; k++) for(int j = 1; j <= NY; j++) for(int i = 0; i <= NX + 1; i++){
+    uy1[SC(i, j, k)] += pow(UY(i, j, k), 1.);
+    uy2[SC(i, j, k)] += pow(UY(i, j, k), 2.);
+  }
+  /* collect_mean_uz, :169-188 */
+  for(int k = 1; k <= NZ; k++) for(int j = 1; j <= NY; j++) for(int i = 0; i <= NX + 1; i++){
+    uz1[SC(i, j, k)] += pow(UZ(i, j, k), 1.);
+    uz2[SC(i, j, k)] += pow(UZ(i, j, k), 2.);
+  }
+  /* collect_adv, :190-207 */
+  for(int k = 1; k <= NZ; k++) for(int j = 1; j <= NY; j++) for(int i = 1; i <= NX + 1; i++){
+    const double x = 0.5 * UX(i, j - 1, k) + 0.5 * UX(i, j, k);
+    const double y = 0.5 * UY(i - 1, j, k) + 0.5 * UY(i, j, k);
+    adv[SF(i, j, k)] += x * y;
+  }
+  /* collect_dif, :209-226 */
+  for(int k = 1; k <= NZ; k++) for(int j = 1; j <= NY; j++) for(int i = 1; i <= NX + 1; i++){
+    dif[SF(i, j, k)] += LYX0(i, j, k) + LYX1(i, j, k) + LXY(i, j, k);
+  }
+  o->st_num += 1;
+  return o->st_num;
+}
+

@@ -67,6 +67,9 @@ double orc_divergence(const orc_t * o);                 /* logging/divergence.c 
 void orc_injection(const orc_t * o, double out[2]);     /* logging/injection.c, raw sums x diffusivity */
 void orc_dissipation(const orc_t * o, double out[2]);   /* logging/dissipation.c */
 void orc_total_energy(const orc_t * o, double out[4]);  /* logging/total_energy.c */
+/* statistics.c collect: adds one sample to ux1 ux2 uy1 uy2 uz1 uz2 adv dif (orc_array names),
+ * returns the sample count */
+size_t orc_stat_collect(orc_t * o);
 
 /* unit-level pieces, exposed so the tests can pin them on their own */
 void orc_fft_c2c(int n, int sign, const double * in, double * out);  /* interleaved complex */

@@ -65,6 +65,8 @@ def library():
     for name in ("orc_gtsv", "orc_gtsv_complex", "orc_tdm_periodic"):
         getattr(lib, name).argtypes = [ctypes.c_int, dp, dp, dp, dp]
     lib.orc_rkcoefs.argtypes = [dp]
+    lib.orc_stat_collect.argtypes = [vp]
+    lib.orc_stat_collect.restype = ctypes.c_size_t
     _LIB = lib
     return lib
 
@@ -81,6 +83,10 @@ def _shape(name, nx, ny, nz):
         return (nz + 2, ny + 2, nx + 2)
     if name in ("lxx", "lyy0", "lyy1", "lzy", "lyz", "lzz"):
         return (nz + 2, ny + 2, nx)
+    if name in ("ux1", "ux2", "adv", "dif"):
+        return (nz, ny, nx + 1)
+    if name in ("uy1", "uy2", "uz1", "uz2"):
+        return (nz, ny, nx + 2)
     if name.startswith("srcux"):
         return (nz, ny, nx - 1)
     if name.startswith("src"):
@@ -200,6 +206,13 @@ class Oracle:
         out = np.zeros(4)
         self._lib.orc_total_energy(self._h, _dp(out))
         return out
+
+    def stat_collect(self):
+        """statistics.c collect (the dif term reads the tensors of the last predict)"""
+        return self._lib.orc_stat_collect(self._h)
+
+    def stats(self):
+        return {n: self.array(n).copy() for n in ("ux1", "ux2", "uy1", "uy2", "uz1", "uz2", "adv", "dif")}
 
 
 def fft_c2c(x, sign):

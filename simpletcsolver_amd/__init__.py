@@ -23,8 +23,12 @@ EXPORTED = (
     "tc_decide_dt", "tc_predict", "tc_potential", "tc_correct", "tc_pressure",
     "tc_divergence", "tc_injection", "tc_dissipation", "tc_total_energy",
     "tc_profile_enable", "tc_profile_read", "tc_device_bytes", "tc_ic_grid", "tc_ic_fields",
-    "tc_arm_diagnostics", "tc_probe_copy_bandwidth",
+    "tc_arm_diagnostics", "tc_probe_copy_bandwidth", "tc_set_field_ref", "tc_get_field_ref",
+    "tc_set_dt_params", "tc_stat_arm", "tc_stat_collect", "tc_stat_reset", "tc_stat_get",
 )
+
+# running statistics of statistics.c, in its accumulation order
+STATS = ("ux1", "ux2", "uy1", "uy2", "uz1", "uz2", "adv", "dif")
 
 
 class TcParams(ctypes.Structure):
@@ -84,6 +88,13 @@ def load_library(path=None):
     lib.tc_ic_grid.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, dp, dp]
     lib.tc_ic_fields.argtypes = [vp, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, ctypes.c_double]
+    lib.tc_set_field_ref.argtypes = [vp, ctypes.c_char_p, dp]
+    lib.tc_get_field_ref.argtypes = [vp, ctypes.c_char_p, dp]
+    lib.tc_set_dt_params.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.tc_stat_arm.argtypes = [vp, ctypes.c_int]
+    lib.tc_stat_collect.argtypes = [vp]
+    lib.tc_stat_reset.argtypes = [vp]
+    lib.tc_stat_get.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_size_t)]
     if path is None:
         _LIB = lib
     return lib
@@ -109,6 +120,17 @@ def ic_grid(nx, lx, is_curved, stretch=2.0):
 
 def npy_shape(name, nx, ny, nz):
     return (nz, ny, nx + 1) if name == "ux" else (nz, ny, nx + 2)
+
+
+def ref_shape(name, nx, ny, nz):
+    """shape of the reference's host array of a field (array_macros/fluid/*.h): y and z
+    halos included"""
+    return (nz + 2, ny + 2, nx + 1) if name == "ux" else (nz + 2, ny + 2, nx + 2)
+
+
+def stat_shape(name, nx, ny, nz):
+    """shape of a statistics accumulator (array_macros/statistics/*.h)"""
+    return (nz, ny, nx + 1) if name in ("ux1", "ux2", "adv", "dif") else (nz, ny, nx + 2)
 
 
 class Solver:
@@ -250,3 +272,38 @@ class Solver:
 
     def device_bytes(self):
         return self._lib.tc_device_bytes(self._h)
+
+    def set_dt_params(self, coef_dt_adv, coef_dt_dif, dt_fixed=0.0):
+        self._c(self._lib.tc_set_dt_params(self._h, float(coef_dt_adv), float(coef_dt_dif),
+                                           float(dt_fixed)), "tc_set_dt_params")
+
+    def set_field_ref(self, name, a):
+        """replace a field from the reference's host layout (y and z halos included)"""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != ref_shape(name, self.nx, self.ny, self.nz):
+            raise ValueError(f"{name}: shape {a.shape}")
+        self._c(self._lib.tc_set_field_ref(self._h, name.encode(), _dp(a)), "tc_set_field_ref")
+
+    def get_field_ref(self, name):
+        a = np.zeros(ref_shape(name, self.nx, self.ny, self.nz))
+        self._c(self._lib.tc_get_field_ref(self._h, name.encode(), _dp(a)), "tc_get_field_ref")
+        return a
+
+    def stat_arm(self, on=True):
+        self._c(self._lib.tc_stat_arm(self._h, int(bool(on))), "tc_stat_arm")
+
+    def stat_collect(self):
+        self._c(self._lib.tc_stat_collect(self._h), "tc_stat_collect")
+
+    def stat_reset(self):
+        self._c(self._lib.tc_stat_reset(self._h), "tc_stat_reset")
+
+    def stats(self):
+        """the accumulators and the sample count"""
+        out = {}
+        num = ctypes.c_size_t(0)
+        for name in STATS:
+            a = np.zeros(stat_shape(name, self.nx, self.ny, self.nz))
+            self._c(self._lib.tc_stat_get(self._h, name.encode(), _dp(a), ctypes.byref(num)), "tc_stat_get")
+            out[name] = a
+        return out, num.value

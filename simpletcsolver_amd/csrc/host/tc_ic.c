@@ -11,8 +11,9 @@
  *            key = seed * 0x9E3779B97F4A7C15 + (field << 56) + linear NPY index
  *   modes    mode_amp X(x) Y(y) Z(z), each factor the mean of nmodes seeded sine/cosine
  *            modes (vanishing at the walls in x), a smooth "turbulent-like" start
- * Each field is assembled in a host buffer in the NPY layout and uploaded with
- * tc_set_field, so the same values reach the device as a file written by the tool would.
+ * The per-index factors (profile, modes) are evaluated on the host with the C library's
+ * transcendental functions; the device kernel adds the hash noise and the mode product in a
+ * fixed operand order, so the field equals one a host loop would write, bit for bit.
  */
 #include <stdio.h>
 #include <stdlib.h>

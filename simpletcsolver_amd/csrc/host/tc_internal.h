@@ -64,6 +64,11 @@ struct tc_solver_s {
   double diag_vals[10];
   int diag_valid, diag_pending, diag_armed;
   int dt_valid;
+  /* running statistics: ux1 ux2 uy1 uy2 uz1 uz2 adv dif, then the tensor sum of the last
+   * substep (allocated on first use) */
+  double * d_stat[9];
+  size_t stat_num;
+  int stat_armed;
   /* bookkeeping */
   void * allocs[TC_MAXALLOC];
   int nalloc;

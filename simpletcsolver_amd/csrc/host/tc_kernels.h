@@ -57,5 +57,8 @@ int tck_launch_diag_cells(const tc_geom_t * g, const tc_coef_t * c, const double
     double * out, hipStream_t stream);
 int tck_launch_diag_walls(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
     const double * uy, const double * t, int scalar, double * part, double * out, hipStream_t stream);
+/* what 0: acc[8] = LYX0 + LYX1 + LXY of (ux, uy); what 1: add the samples to acc[0..7] */
+int tck_launch_stat(const tc_geom_t * g, const tc_coef_t * c, int what, const double * ux,
+    const double * uy, const double * uz, double * const acc[9], hipStream_t stream);
 
 #endif

@@ -467,7 +467,9 @@ static void npy_x(const tc_solver_t * s, int f, int * nxs, int * i0){
   else { *nxs = s->p.nx + 2; *i0 = 0; }
 }
 
-int tc_set_field(tc_solver_t * s, const char * name, const double * data){
+/* layout 0: NPY (nz, ny, nxs); layout 1: the reference's host array with one halo plane on
+ * each side in y and z, (nz+2, ny+2, nxs) (include/array_macros/fluid/*.h) */
+static int set_field_layout(tc_solver_t * s, const char * name, const double * data, int layout){
   const int f = field_id(name);
   if(f < 0) return tc_fail("unknown field '%s'", name);
   HIPOK(hipSetDevice(s->p.device));
@@ -476,12 +478,21 @@ int tc_set_field(tc_solver_t * s, const char * name, const double * data){
   const tc_geom_t * g = &s->g;
   double * dst = tc_field_ptr(s, f);
   for(int k = 1; k <= g->nz; k++){
-    const double * src = data + (size_t)(k - 1) * g->ny * nxs;
+    const double * src = layout == 0 ? data + (size_t)(k - 1) * g->ny * nxs
+                                     : data + (size_t)nxs * (1 + (size_t)(g->ny + 2) * k);
     double * d = dst + (size_t)(i0 + TC_XOFF) + (size_t)g->px * (1 + (size_t)g->py * k);
     HIPOK(hipMemcpy2DAsync(d, sizeof(double) * g->px, src, sizeof(double) * nxs,
         sizeof(double) * nxs, g->ny, hipMemcpyHostToDevice, s->stream));
   }
   return tc_finish_field(s, f);
+}
+
+int tc_set_field(tc_solver_t * s, const char * name, const double * data){
+  return set_field_layout(s, name, data, 0);
+}
+
+int tc_set_field_ref(tc_solver_t * s, const char * name, const double * data){
+  return set_field_layout(s, name, data, 1);
 }
 
 /* after the interior of a field was replaced: boundary values and halos, the second buffer
@@ -499,7 +510,7 @@ int tc_finish_field(tc_solver_t * s, int f){
   return 0;
 }
 
-int tc_get_field(tc_solver_t * s, const char * name, double * data){
+static int get_field_layout(tc_solver_t * s, const char * name, double * data, int layout){
   const int f = field_id(name);
   if(f < 0) return tc_fail("unknown field '%s'", name);
   HIPOK(hipSetDevice(s->p.device));
@@ -507,13 +518,50 @@ int tc_get_field(tc_solver_t * s, const char * name, double * data){
   npy_x(s, f, &nxs, &i0);
   const tc_geom_t * g = &s->g;
   const double * src = tc_field_ptr(s, f);
-  for(int k = 1; k <= g->nz; k++){
-    double * d = data + (size_t)(k - 1) * g->ny * nxs;
-    const double * sp = src + (size_t)(i0 + TC_XOFF) + (size_t)g->px * (1 + (size_t)g->py * k);
-    HIPOK(hipMemcpy2DAsync(d, sizeof(double) * nxs, sp, sizeof(double) * g->px,
-        sizeof(double) * nxs, g->ny, hipMemcpyDeviceToHost, s->stream));
+  if(layout == 0){
+    for(int k = 1; k <= g->nz; k++){
+      double * d = data + (size_t)(k - 1) * g->ny * nxs;
+      const double * sp = src + (size_t)(i0 + TC_XOFF) + (size_t)g->px * (1 + (size_t)g->py * k);
+      HIPOK(hipMemcpy2DAsync(d, sizeof(double) * nxs, sp, sizeof(double) * g->px,
+          sizeof(double) * nxs, g->ny, hipMemcpyDeviceToHost, s->stream));
+    }
+  }else{
+    /* every row of every plane, halos included */
+    const double * sp = src + (size_t)(i0 + TC_XOFF);
+    HIPOK(hipMemcpy2DAsync(data, sizeof(double) * nxs, sp, sizeof(double) * g->px,
+        sizeof(double) * nxs, (size_t)g->py * g->pz, hipMemcpyDeviceToHost, s->stream));
   }
   HIPOK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int tc_get_field(tc_solver_t * s, const char * name, double * data){
+  return get_field_layout(s, name, data, 0);
+}
+
+int tc_get_field_ref(tc_solver_t * s, const char * name, double * data){
+  return get_field_layout(s, name, data, 1);
+}
+
+/* decide_dt coefficients (decide_dt.c:128-129) and the fixed step */
+int tc_set_dt_params(tc_solver_t * s, double coef_dt_adv, double coef_dt_dif, double dt_fixed){
+  s->p.coef_dt_adv = coef_dt_adv;
+  s->p.coef_dt_dif = coef_dt_dif;
+  s->p.dt_fixed = dt_fixed;
+  const double diffusivity = fmax(s->nu, s->kappa);
+  const int nx = s->p.nx;
+  double grid[3];
+  grid[0] = DBL_MAX;
+  for(int i = 0; i <= nx + 1; i++) grid[0] = fmin(grid[0], s->h_tab[(size_t)T_HXXC * s->tp + i]);
+  grid[1] = s->h_tab[(size_t)T_HYXC * s->tp + 1];
+  grid[2] = s->hz;
+  double lim = INFINITY;
+  for(int d = 0; d < 3; d++){
+    const double dif = coef_dt_dif / diffusivity * 0.5 / 3 * pow(grid[d], 2.);
+    if(!s->p.implicit[d]) lim = fmin(lim, dif);
+  }
+  s->dif_limit = lim;
+  s->dt_valid = 0;
   return 0;
 }
 
@@ -656,6 +704,11 @@ static int predict(tc_solver_t * s, int rkstep){
   const int any_implicit = s->p.implicit[0] || s->p.implicit[1] || s->p.implicit[2];
   if(s->diag_armed && rkstep == 2){
     if(snapshot_diag(s)) return 1;
+  }
+  if(s->stat_armed && rkstep == 2){
+    /* the tensors that statistics.c:221-234 reads are those of the last substep */
+    PROF(TC_S_DIAG, tck_launch_stat(&s->g, &s->coef, 0, tc_field_ptr(s, TC_FUX),
+        tc_field_ptr(s, TC_FUY), tc_field_ptr(s, TC_FUZ), s->d_stat, s->stream));
   }
   const double * cur[4];
   double * nxt[4];
@@ -873,6 +926,62 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
 
 int tc_arm_diagnostics(tc_solver_t * s, int on){
   s->diag_armed = on;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
+ * running statistics (src/statistics.c:56-307) on the device
+ * ------------------------------------------------------------------------------------- */
+static const char * const g_stat_names[8] = {"ux1", "ux2", "uy1", "uy2", "uz1", "uz2", "adv", "dif"};
+
+static size_t stat_count(const tc_solver_t * s, int n){
+  const size_t rows = (size_t)s->p.ny * (size_t)s->p.nz;
+  const int face = (n == 0 || n == 1 || n == 6 || n == 7 || n == 8);
+  return rows * (size_t)(face ? s->p.nx + 1 : s->p.nx + 2);
+}
+
+int tc_stat_arm(tc_solver_t * s, int on){
+  HIPOK(hipSetDevice(s->p.device));
+  if(on && !s->d_stat[0]){
+    for(int n = 0; n < 9; n++){
+      if(dalloc(s, &s->d_stat[n], stat_count(s, n))) return 1;
+    }
+    s->stat_num = 0;
+  }
+  s->stat_armed = on;
+  return 0;
+}
+
+int tc_stat_reset(tc_solver_t * s){
+  if(!s->d_stat[0]) return 0;
+  HIPOK(hipSetDevice(s->p.device));
+  for(int n = 0; n < 8; n++){
+    HIPOK(hipMemsetAsync(s->d_stat[n], 0, sizeof(double) * stat_count(s, n), s->stream));
+  }
+  s->stat_num = 0;
+  return 0;
+}
+
+int tc_stat_collect(tc_solver_t * s){
+  if(!s->d_stat[0]) return tc_fail("tc_stat_collect: statistics are not armed (tc_stat_arm)");
+  HIPOK(hipSetDevice(s->p.device));
+  KOK(tck_launch_stat(&s->g, &s->coef, 1, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
+      tc_field_ptr(s, TC_FUZ), s->d_stat, s->stream));
+  s->stat_num += 1;
+  return 0;
+}
+
+int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num){
+  if(!s->d_stat[0]) return tc_fail("tc_stat_get: statistics are not armed (tc_stat_arm)");
+  int n = -1;
+  for(int m = 0; m < 8; m++) if(name && strcmp(name, g_stat_names[m]) == 0) n = m;
+  if(n < 0) return tc_fail("tc_stat_get: unknown statistic '%s'", name ? name : "(null)");
+  HIPOK(hipSetDevice(s->p.device));
+  if(host){
+    HIPOK(hipMemcpyAsync(host, s->d_stat[n], sizeof(double) * stat_count(s, n), hipMemcpyDeviceToHost, s->stream));
+    HIPOK(hipStreamSynchronize(s->stream));
+  }
+  if(num) *num = s->stat_num;
   return 0;
 }
 

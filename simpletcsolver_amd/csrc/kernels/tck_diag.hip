@@ -268,6 +268,65 @@ __global__ void __launch_bounds__(256) tck_diag_finish(const double * part, long
 }
 
 /* ------------------------------------------------------------------------------------- */
+/* running statistics, src/statistics.c:111-234.  The accumulators use the reference's
+ * compact layouts: ux1, ux2, adv, dif over i = 1..nx+1 and uy1, uy2, uz1, uz2 over
+ * i = 0..nx+1, then j = 1..ny, k = 1..nz, i fastest. */
+struct StatArgs {
+  tc_geom_t g;
+  tc_coef_t c;
+  const double * ux; const double * uy; const double * uz;
+  double * ux1; double * ux2; double * uy1; double * uy2; double * uz1; double * uz2;
+  double * adv; double * dif;
+  const double * difs;   /* LYX0 + LYX1 + LXY of the tensors the last substep formed */
+};
+
+/* difs(i, j, k) = LYX0 + LYX1 + LXY (predict/lyx.c:24-95, predict/lxy.c:19-49) from the
+ * velocity of the start of the last substep; i = 1..nx+1 */
+__global__ void __launch_bounds__(TCK_BLOCK) tck_stat_tensors(const StatArgs A){
+  const tc_geom_t & g = A.g;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if(i > g.nx + 1 || j > g.ny) return;
+  const long long o = tck_idx(g, i, j, k);
+  const double * __restrict__ ux = A.ux;
+  const double * __restrict__ uy = A.uy;
+  const double lyx0 = 1. / A.c.hyxf[i] * (- ux[o - g.px] + ux[o]);
+  const double lyx1 = A.c.l1f[i] * (
+      + (1 == i ? 0. : 0.5) * uy[o - 1]
+      + (g.nx + 1 == i ? 0. : 0.5) * uy[o]);
+  const double lxy = 1. / A.c.hxxf[i] * (- uy[o - 1] + uy[o]);
+  const long long q = (i - 1) + (long long)(g.nx + 1) * ((j - 1) + (long long)g.ny * (k - 1));
+  ((double *)A.difs)[q] = lyx0 + lyx1 + lxy;
+}
+
+__global__ void __launch_bounds__(TCK_BLOCK) tck_stat_collect(const StatArgs A){
+  const tc_geom_t & g = A.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;    /* 0 .. nx+1 */
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if(i > g.nx + 1 || j > g.ny) return;
+  const long long o = tck_idx(g, i, j, k);
+  const long long jk = (j - 1) + (long long)g.ny * (k - 1);
+  const long long qc = i + (long long)(g.nx + 2) * jk;
+  const double vy = A.uy[o], vz = A.uz[o];
+  A.uy1[qc] += vy;
+  A.uy2[qc] += vy * vy;
+  A.uz1[qc] += vz;
+  A.uz2[qc] += vz * vz;
+  if(i >= 1){
+    const long long qf = (i - 1) + (long long)(g.nx + 1) * jk;
+    const double vx = A.ux[o];
+    A.ux1[qf] += vx;
+    A.ux2[qf] += vx * vx;
+    const double x = 0.5 * A.ux[o - g.px] + 0.5 * vx;
+    const double y = 0.5 * A.uy[o - 1] + 0.5 * vy;
+    A.adv[qf] += x * y;
+    A.dif[qf] += A.difs[qf];
+  }
+}
+
+/* ------------------------------------------------------------------------------------- */
 extern "C" {
 
 int tck_launch_bc(const tc_geom_t * g, double * a, int kind, double wm, double wp, hipStream_t stream){
@@ -307,6 +366,21 @@ int tck_launch_diag_walls(const tc_geom_t * g, const tc_coef_t * c, const double
   hipLaunchKernelGGL(tck_diag_walls, dim3((g->ny + TCK_BLOCK - 1) / TCK_BLOCK, g->nz), dim3(TCK_BLOCK), 0, stream, A);
   TCK_CHECK_LAUNCH();
   hipLaunchKernelGGL(tck_diag_finish, dim3(1), dim3(256), 0, stream, part, tck_diag_walls_blocks(g), 2, 0, out);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
+int tck_launch_stat(const tc_geom_t * g, const tc_coef_t * c, int what, const double * ux,
+    const double * uy, const double * uz, double * const acc[9], hipStream_t stream){
+  StatArgs A;
+  A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.uz = uz;
+  A.ux1 = acc[0]; A.ux2 = acc[1]; A.uy1 = acc[2]; A.uy2 = acc[3];
+  A.uz1 = acc[4]; A.uz2 = acc[5]; A.adv = acc[6]; A.dif = acc[7]; A.difs = acc[8];
+  if(what == 0){
+    hipLaunchKernelGGL(tck_stat_tensors, dim3((g->nx + 1 + 63) / 64, (g->ny + 3) / 4, g->nz), dim3(64, 4), 0, stream, A);
+  }else{
+    hipLaunchKernelGGL(tck_stat_collect, dim3((g->nx + 2 + 63) / 64, (g->ny + 3) / 4, g->nz), dim3(64, 4), 0, stream, A);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }

@@ -360,7 +360,7 @@ static size_t g_lds_limit = 0;
 static size_t lds_limit(void){
   if(g_lds_limit == 0){
     int dev = 0;
-    hipGetDevice(&dev);
+    if(hipGetDevice(&dev) != hipSuccess) dev = 0;
     int v = 0;
     if(hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || v <= 0){
       if(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;

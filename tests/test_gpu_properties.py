@@ -1,0 +1,112 @@
+"""Device path against what the mathematics fixes and against the oracle's statistics:
+
+* a manufactured potential: u = gamma dt grad(psi_m) (the gradient of correct/*.c) must come
+  back from tc_potential as psi_m up to a constant, at sizes where the FFT lengths take every
+  radix of the device FFT (2, 3, 4, 8) and an odd radial length;
+* a step leaves a solenoidal field (max |div u| at round-off, logging/divergence.c);
+* plane Couette stays an exact steady state;
+* running statistics (statistics.c) equal the oracle's after steps and samples in between;
+* restart through the host copies is bit-exact on the device.
+"""
+import numpy as np
+import pytest
+
+from harness import gpu_for, make_case, oracle_for, rel_l2
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("curved", [True, False])
+@pytest.mark.parametrize("shape", [(16, 12, 8), (9, 24, 6), (33, 16, 48)])
+def test_device_manufactured_potential(need_gpu, curved, shape):
+    nx, ny, nz = shape
+    case = make_case(nx=nx, ny=ny, nz=nz, Re=100.0, is_curved=curved, amp=0.0, uy_walls=(0.0, 0.0))
+    o = oracle_for(case)
+    hxxf, hyxc, hz = o.array("hxxf").copy(), o.array("hyxc").copy(), o.hz
+    o.close()
+    rng = np.random.default_rng(23)
+    psi = rng.standard_normal((nz, ny, nx))
+    rk, dt = 2, 0.21
+    g = orc.rkcoefs()[rk, 2] * dt
+    ux = np.zeros((nz, ny, nx + 1))
+    ux[:, :, 1:nx] = g * (psi[:, :, 1:] - psi[:, :, :-1]) / hxxf[1:nx]
+    uy = np.zeros((nz, ny, nx + 2))
+    uy[:, :, 1:nx + 1] = g * (psi - np.roll(psi, 1, axis=1)) / hyxc[1:nx + 1]
+    uz = np.zeros((nz, ny, nx + 2))
+    uz[:, :, 1:nx + 1] = g * (psi - np.roll(psi, 1, axis=0)) / hz
+    s = gpu_for(case)
+    s.load_fields({"ux": ux, "uy": uy, "uz": uz})
+    s.potential(rk, dt)
+    got = s.get_field("psi")[:, :, 1:-1]
+    d = got - psi
+    err = np.abs(d - d.mean()).max() / np.abs(psi).max()
+    print(f"manufactured psi {shape} curved={curved}: {err:.3e}")
+    assert err <= 1e-11
+    s.close()
+
+
+@pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 1, 1)])
+def test_device_step_is_solenoidal(need_gpu, implicit):
+    case = make_case(nx=24, ny=16, nz=12, Re=300.0, implicit=implicit, amp=0.3)
+    s = gpu_for(case)
+    assert s.divergence() > 1e-3
+    for _ in range(3):
+        s.step()
+        assert s.divergence() <= 1e-13
+    s.close()
+
+
+def test_device_plane_couette_steady(need_gpu):
+    case = make_case(nx=32, ny=8, nz=8, Re=50.0, is_curved=False, implicit=(1, 1, 1), amp=0.0,
+                     stretch=2.5)
+    s = gpu_for(case)
+    u0 = s.fields()
+    for _ in range(5):
+        s.step()
+    u1 = s.fields()
+    for n in ("ux", "uy", "uz", "t"):
+        assert np.abs(u1[n] - u0[n]).max() <= 1e-13, n
+    s.close()
+
+
+@pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 0, 0)])
+def test_statistics_match_oracle(need_gpu, implicit):
+    case = make_case(nx=16, ny=12, nz=8, Re=200.0, implicit=implicit, amp=0.2, t_amp=0.05)
+    o, s = oracle_for(case), gpu_for(case)
+    s.stat_arm(True)
+    for step in range(4):
+        o.integrate()
+        s.step()
+        if step % 2 == 1:
+            o.stat_collect()
+            s.stat_collect()
+    got, num = s.stats()
+    want = o.stats()
+    assert num == 2
+    for n, a in want.items():
+        e = rel_l2(got[n], a)
+        print(f"{n}: {e:.3e}")
+        assert e <= 1e-10, n
+    s.close()
+
+
+def test_device_restart_is_bit_exact(need_gpu):
+    case = make_case(nx=16, ny=12, nz=8, Re=200.0, implicit=(1, 1, 1), amp=0.2, t_amp=0.05)
+    s = gpu_for(case)
+    for _ in range(4):
+        s.step()
+    full = s.fields()
+    s.close()
+    s = gpu_for(case)
+    for _ in range(2):
+        s.step()
+    mid = s.fields()
+    s.close()
+    s = gpu_for(dict(case, fields=mid))
+    for _ in range(2):
+        s.step()
+    again = s.fields()
+    for n in ("ux", "uy", "uz", "p", "t"):
+        assert np.array_equal(full[n], again[n]), n
+    s.close()
