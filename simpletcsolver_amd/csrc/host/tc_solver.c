@@ -325,11 +325,11 @@ static int setup_linesolve(tc_solver_t * s){
       }
       if(s->p.implicit[1]){
         if(dalloc(s, &L->yr, (size_t)ny * tp) || dalloc(s, &L->yv, (size_t)ny * tp) ||
-           dalloc(s, &L->yq1, (size_t)ny * tp) || dalloc(s, &L->ymeta, (size_t)3 * tp)) return 1;
+           dalloc(s, &L->yq1, (size_t)ny * tp) || dalloc(s, &L->ymeta, (size_t)7 * tp)) return 1;
       }
       if(s->p.implicit[2]){
         if(dalloc(s, &L->zr, nz) || dalloc(s, &L->zv, nz) || dalloc(s, &L->zq1, nz) ||
-           dalloc(s, &L->zmeta, 3)) return 1;
+           dalloc(s, &L->zmeta, 7)) return 1;
       }
     }
   }

@@ -58,23 +58,36 @@ __device__ void factor_sys(int n, const double * lc, const double * cc, const do
 }
 
 /* constant-coefficient periodic system of size n (tdm.c:78-99): factors of the leading
- * (n-1) block, the correction vector q1 and the Sherman-Morrison denominator */
+ * (n-1) block, the correction vector q1 and the meta row {den, l, u, jc, rc, vc, rlast}.
+ * The factor recurrence r_i = 1 / (c - l v_{i-1}), v_i = r_i u of a constant-coefficient
+ * system is a deterministic map of v_{i-1}: once two consecutive rows are equal bit for bit,
+ * every later row (up to the last, which has its own pivot rule) repeats them.  jc is the
+ * first such row; solvers read rows below jc from the tables and use rc, vc above. */
+#define TC_PMETA 7
+
 __device__ void factor_periodic(int n, double tl, double tc, double tu, long long stride,
-    double * r, double * v, double * q1, double * den){
+    double * r, double * v, double * q1, double * meta, long long mstride){
   const int m = n - 1;
   /* factors of the (n-1) leading block */
   double vprev = tu / tc;
   r[0] = tc;
   v[0] = vprev;
+  int jc = m - 1;
+  double rprev = 0.;
   for(int i = 1; i < m - 1; i++){
     const double val = 1. / (tc - tl * vprev);
-    vprev = val * (tu);
+    const double vi = val * (tu);
+    if(jc == m - 1 && i >= 2 && val == rprev && vi == vprev) jc = i;
     r[stride * i] = val;
-    v[stride * i] = vprev;
+    v[stride * i] = vi;
+    rprev = val;
+    vprev = vi;
   }
+  double rlast;
   {
     const double val = tc - tl * vprev;
-    r[stride * (m - 1)] = fabs(val) > DBL_EPSILON ? 1. / val : 0.;
+    rlast = fabs(val) > DBL_EPSILON ? 1. / val : 0.;
+    r[stride * (m - 1)] = rlast;
     v[stride * (m - 1)] = 0.;
   }
   /* q1 = A_{n-1}^{-1} (-l0, 0, ..., 0, -u_{n-2}) by the same gtsv arithmetic */
@@ -90,7 +103,13 @@ __device__ void factor_periodic(int n, double tl, double tc, double tu, long lon
   for(int i = m - 2; i >= 0; i--){
     q1[stride * i] -= v[stride * i] * q1[stride * (i + 1)];
   }
-  *den = tc + tu * q1[0] + tl * q1[stride * (m - 1)];
+  meta[0] = tc + tu * q1[0] + tl * q1[stride * (m - 1)];
+  meta[mstride] = tl;
+  meta[2 * mstride] = tu;
+  meta[3 * mstride] = (double)jc;
+  meta[4 * mstride] = jc < m - 1 ? r[stride * jc] : 0.;
+  meta[5 * mstride] = jc < m - 1 ? v[stride * jc] : 0.;
+  meta[6 * mstride] = rlast;
 }
 
 /* implicit-diffusion factor tables, one job per (field, substep).
@@ -99,7 +118,7 @@ __device__ void factor_periodic(int n, double tl, double tc, double tu, long lon
  *   z: one periodic system of size nz (stencil lapz)
  * prefactor = fac * (0.5 * gamma * dt * diffusivity) (ux.c:496-506 and its siblings; fac is
  * 2 for the diagonal stress direction of each velocity and for T in y, t.c:346-349).
- * ymeta/zmeta hold {den, l, u} of the periodic systems ([3][tp] and [3]).  All jobs of a
+ * ymeta/zmeta hold the TC_PMETA meta rows of the periodic systems ([7][tp] and [7]).  All jobs of a
  * step run in one launch: each system is a short sequential recurrence, so the launch costs
  * one recurrence latency however many systems it holds. */
 #define FACTOR_MAXJOB 12
@@ -146,22 +165,14 @@ __global__ void tck_factor(const FactorBatch B){
     const double tl = - pf * A.lyl[i];
     const double tc = 1. - pf * A.lyc[i];
     const double tu = - pf * A.lyu[i];
-    double den;
-    factor_periodic(B.ny, tl, tc, tu, B.tp, A.yr + i, A.yv + i, A.yq1 + i, &den);
-    A.ymeta[i] = den;
-    A.ymeta[B.tp + i] = tl;
-    A.ymeta[2 * B.tp + i] = tu;
+    factor_periodic(B.ny, tl, tc, tu, B.tp, A.yr + i, A.yv + i, A.yq1 + i, A.ymeta + i, B.tp);
   }
   if(t == B.nx + 1 && B.dirs[2]){
     const double pf = A.fac[2] * prefactor;
     const double tl = - pf * B.lzl;
     const double tc = 1. - pf * B.lzc;
     const double tu = - pf * B.lzu;
-    double den;
-    factor_periodic(B.nz, tl, tc, tu, 1, A.zr, A.zv, A.zq1, &den);
-    A.zmeta[0] = den;
-    A.zmeta[1] = tl;
-    A.zmeta[2] = tu;
+    factor_periodic(B.nz, tl, tc, tu, 1, A.zr, A.zv, A.zq1, A.zmeta, 1);
   }
 }
 
@@ -169,108 +180,148 @@ __global__ void tck_factor(const FactorBatch B){
  * wave-level scans of affine maps x -> a x + b
  * ------------------------------------------------------------------------------------- */
 /* value entering this lane's segment when maps compose in increasing lane order, x0 = 0 */
-__device__ __forceinline__ void scan_fwd(double a, double * b, int nb, double * in){
+template <int NC>
+__device__ __forceinline__ void scan_fwd(double a, double * b, double * in){
   const int lane = threadIdx.x % TCK_WAVE;
+  #pragma unroll
   for(int off = 1; off < TCK_WAVE; off <<= 1){
     const double ap = __shfl_up(a, off, TCK_WAVE);
-    double bp[2];
-    for(int q = 0; q < nb; q++) bp[q] = __shfl_up(b[q], off, TCK_WAVE);
+    double bp[NC];
+    #pragma unroll
+    for(int q = 0; q < NC; q++) bp[q] = __shfl_up(b[q], off, TCK_WAVE);
     if(lane >= off){
-      for(int q = 0; q < nb; q++) b[q] = a * bp[q] + b[q];
+      #pragma unroll
+      for(int q = 0; q < NC; q++) b[q] = a * bp[q] + b[q];
       a = a * ap;
     }
   }
-  for(int q = 0; q < nb; q++){
+  #pragma unroll
+  for(int q = 0; q < NC; q++){
     const double prev = __shfl_up(b[q], 1, TCK_WAVE);
     in[q] = lane == 0 ? 0. : prev;
   }
 }
 
 /* same in decreasing lane order */
-__device__ __forceinline__ void scan_bwd(double a, double * b, int nb, double * in){
+template <int NC>
+__device__ __forceinline__ void scan_bwd(double a, double * b, double * in){
   const int lane = threadIdx.x % TCK_WAVE;
+  #pragma unroll
   for(int off = 1; off < TCK_WAVE; off <<= 1){
     const double ap = __shfl_down(a, off, TCK_WAVE);
-    double bp[2];
-    for(int q = 0; q < nb; q++) bp[q] = __shfl_down(b[q], off, TCK_WAVE);
+    double bp[NC];
+    #pragma unroll
+    for(int q = 0; q < NC; q++) bp[q] = __shfl_down(b[q], off, TCK_WAVE);
     if(lane + off < TCK_WAVE){
-      for(int q = 0; q < nb; q++) b[q] = a * bp[q] + b[q];
+      #pragma unroll
+      for(int q = 0; q < NC; q++) b[q] = a * bp[q] + b[q];
       a = a * ap;
     }
   }
-  for(int q = 0; q < nb; q++){
+  #pragma unroll
+  for(int q = 0; q < NC; q++){
     const double next = __shfl_down(b[q], 1, TCK_WAVE);
     in[q] = lane == TCK_WAVE - 1 ? 0. : next;
   }
 }
 
-/* Solve one tridiagonal line held by a wavefront.  q[e][0..nc-1]: this lane's rows
- * row0 .. row0+cnt-1 (cnt may be 0).  r, v, l: factor rows as produced by factor_sys,
- * addressed by a per-row pointer offset. */
-template <int NC>
-__device__ __forceinline__ void wave_tdm(double (*q)[NC], const double * rr, const double * vv,
-    const double * ll, int row0, int cnt, int n){
+/* Solve one tridiagonal line held by a wavefront: lane L owns rows L*SEG .. L*SEG+SEG-1
+ * (rows >= n are padding and pass values through).  q[e][c]: the lane's right-hand sides
+ * (NC = 1 real, 2 complex).  rr, ll, vv: the factor rows of the lane's rows (factor_sys
+ * layout: rr of row 0 holds c0 itself), already in registers, so the recurrences below wait
+ * on no memory. */
+template <int SEG, int NC>
+__device__ __forceinline__ void wave_tdm(double (*q)[NC], const double * rr, const double * ll,
+    const double * vv, int n){
+  const int row0 = (threadIdx.x % TCK_WAVE) * SEG;
   /* forward: q_0 = q_0 / c_0, q_i = r_i (q_i - l_i q_{i-1}) */
-  double a = 1., b[2] = {0., 0.};
+  double a = 1., b[NC];
   #pragma unroll
-  for(int e = 0; e < XS_MAXSEG; e++){
-    if(e < cnt){
-      const int row = row0 + e;
+  for(int c = 0; c < NC; c++) b[c] = 0.;
+  #pragma unroll
+  for(int e = 0; e < SEG; e++){
+    const int row = row0 + e;
+    if(row < n){
       if(row == 0){
         a = 0.;
-        for(int c = 0; c < NC; c++) b[c] = q[e][c] / rr[0];
+        #pragma unroll
+        for(int c = 0; c < NC; c++) b[c] = q[e][c] / rr[e];
       }else{
-        const double ri = rr[row];
-        const double ai = - ri * ll[row];
-        for(int c = 0; c < NC; c++) b[c] = ai * b[c] + ri * q[e][c];
+        const double ai = - rr[e] * ll[e];
+        #pragma unroll
+        for(int c = 0; c < NC; c++) b[c] = ai * b[c] + rr[e] * q[e][c];
         a = ai * a;
       }
     }
   }
-  double in[2];
-  scan_fwd(a, b, NC, in);
+  double in[NC];
+  scan_fwd<NC>(a, b, in);
   #pragma unroll
-  for(int e = 0; e < XS_MAXSEG; e++){
-    if(e < cnt){
-      const int row = row0 + e;
+  for(int e = 0; e < SEG; e++){
+    const int row = row0 + e;
+    if(row < n){
       if(row == 0){
-        for(int c = 0; c < NC; c++) q[e][c] = q[e][c] / rr[0];
+        #pragma unroll
+        for(int c = 0; c < NC; c++) q[e][c] = q[e][c] / rr[e];
       }else{
-        const double ri = rr[row];
-        const double li = ll[row];
-        for(int c = 0; c < NC; c++) q[e][c] = ri * (q[e][c] - li * in[c]);
+        #pragma unroll
+        for(int c = 0; c < NC; c++) q[e][c] = rr[e] * (q[e][c] - ll[e] * in[c]);
       }
+      #pragma unroll
       for(int c = 0; c < NC; c++) in[c] = q[e][c];
     }
   }
   /* backward: q_i -= v_i q_{i+1}, q_{n-1} kept */
   a = 1.;
-  b[0] = b[1] = 0.;
   #pragma unroll
-  for(int e = XS_MAXSEG - 1; e >= 0; e--){
-    if(e < cnt){
-      const int row = row0 + e;
+  for(int c = 0; c < NC; c++) b[c] = 0.;
+  #pragma unroll
+  for(int e = SEG - 1; e >= 0; e--){
+    const int row = row0 + e;
+    if(row < n){
       if(row == n - 1){
         a = 0.;
+        #pragma unroll
         for(int c = 0; c < NC; c++) b[c] = q[e][c];
       }else{
-        const double ai = - vv[row];
+        const double ai = - vv[e];
+        #pragma unroll
         for(int c = 0; c < NC; c++) b[c] = ai * b[c] + q[e][c];
         a = ai * a;
       }
     }
   }
-  scan_bwd(a, b, NC, in);
+  scan_bwd<NC>(a, b, in);
   #pragma unroll
-  for(int e = XS_MAXSEG - 1; e >= 0; e--){
-    if(e < cnt){
-      const int row = row0 + e;
+  for(int e = SEG - 1; e >= 0; e--){
+    const int row = row0 + e;
+    if(row < n){
       if(row != n - 1){
-        for(int c = 0; c < NC; c++) q[e][c] -= vv[row] * in[c];
+        #pragma unroll
+        for(int c = 0; c < NC; c++) q[e][c] -= vv[e] * in[c];
       }
+      #pragma unroll
       for(int c = 0; c < NC; c++) in[c] = q[e][c];
     }
   }
+}
+
+/* position of line element idx in the LDS staging area of a wave: lane idx / SEG owns it,
+ * rows padded to SEG + 1 so that the lanes' segment reads fall in different banks */
+template <int SEG>
+__device__ __forceinline__ int stage_pos(int idx){
+  return (idx / SEG) * (SEG + 1) + (idx % SEG);
+}
+
+static int g_num_cus = 0;
+
+static int num_cus(void){
+  if(g_num_cus == 0){
+    int dev = 0, v = 0;
+    if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    g_num_cus = v;
+  }
+  return g_num_cus;
 }
 
 /* x lines of the implicit diffusion solve (solve_in_x, predict/linear_system.c:6-37).
@@ -284,73 +335,82 @@ struct XSolveArgs {
   int i0, n;
 };
 
-/* one x line per wavefront; the line is staged through LDS so that global loads and stores
- * are contiguous across the wave (lane t touches elements t, t+64, ...), while each lane
- * solves its contiguous segment from LDS.  Segment stride m+1 avoids bank aliasing. */
+/* Four waves per block, one x line per wave per iteration; blocks loop over groups of four
+ * lines, so the factor rows of each lane's segment (the same for every line) are loaded once
+ * per block into registers.  Lines are staged through LDS: global loads and stores are
+ * contiguous across the wave (lane t touches elements t, t+64, ...), each lane solves its
+ * contiguous segment. */
+template <int SEG>
 __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, double * __restrict__ dd,
     const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
     const double * __restrict__ lt, double * sm){
   const tc_geom_t & g = A.g;
   const int w = threadIdx.x / TCK_WAVE;
-  const int wave = blockIdx.x * (TCK_BLOCK / TCK_WAVE) + w;
   const int lane = threadIdx.x % TCK_WAVE;
-  const int nlines = g.ny * g.nz;
-  const bool live = wave < nlines;
-  const int j = live ? 1 + wave % g.ny : 1;
-  const int k = live ? 1 + wave / g.ny : 1;
   const int n = A.n;
-  const int seg = (n + TCK_WAVE - 1) / TCK_WAVE;
-  const int ld = seg + 1;
-  const int row0 = lane * seg;
-  int cnt = n - row0;
-  cnt = cnt < 0 ? 0 : (cnt > seg ? seg : cnt);
-  const long long base = tck_idx(g, A.i0, j, k);
-  if(live){
-    for(int idx = lane; idx < n; idx += TCK_WAVE){
-      const int ln = idx / seg;
-      sm[ln * ld + (idx - ln * seg)] = dd[base + idx];
-    }
-  }
-  __syncthreads();
-  double q[XS_MAXSEG][1];
+  const int row0 = lane * SEG;
+  double rr[SEG], ll[SEG], vv[SEG];
   #pragma unroll
-  for(int e = 0; e < XS_MAXSEG; e++) q[e][0] = e < cnt ? sm[lane * ld + e] : 0.;
-  wave_tdm<1>(q, rt, vt, lt, row0, cnt, n);
-  #pragma unroll
-  for(int e = 0; e < XS_MAXSEG; e++){
-    if(e < cnt) sm[lane * ld + e] = q[e][0];
+  for(int e = 0; e < SEG; e++){
+    const int row = row0 + e < n ? row0 + e : n - 1;
+    rr[e] = rt[row];
+    ll[e] = lt[row];
+    vv[e] = vt[row];
   }
-  __syncthreads();
-  if(!live) return;
-  if(ucur){
-    double u[XS_MAXSEG];
-    #pragma unroll
-    for(int e = 0; e < XS_MAXSEG; e++){
-      const int idx = lane + e * TCK_WAVE;
-      u[e] = idx < n ? ucur[base + idx] : 0.;
-    }
-    #pragma unroll
-    for(int e = 0; e < XS_MAXSEG; e++){
-      const int idx = lane + e * TCK_WAVE;
-      if(idx < n){
-        const int ln = idx / seg;
-        tck_store_halo(dd, g, A.i0 + idx, j, k, u[e] + sm[ln * ld + (idx - ln * seg)], false);
+  const long long nlines = (long long)g.ny * g.nz;
+  const int wpb = TCK_BLOCK / TCK_WAVE;
+  for(long long g0 = (long long)blockIdx.x * wpb; g0 < nlines; g0 += (long long)gridDim.x * wpb){
+    const long long line = g0 + w;
+    const bool live = line < nlines;
+    const int j = live ? 1 + (int)(line % g.ny) : 1;
+    const int k = live ? 1 + (int)(line / g.ny) : 1;
+    const long long base = tck_idx(g, A.i0, j, k);
+    double u[SEG];
+    if(live){
+      #pragma unroll
+      for(int e = 0; e < SEG; e++){
+        const int idx = lane + e * TCK_WAVE;
+        if(idx < n) sm[stage_pos<SEG>(idx)] = dd[base + idx];
+      }
+      if(ucur){
+        #pragma unroll
+        for(int e = 0; e < SEG; e++){
+          const int idx = lane + e * TCK_WAVE;
+          u[e] = idx < n ? ucur[base + idx] : 0.;
+        }
       }
     }
-  }else{
-    for(int idx = lane; idx < n; idx += TCK_WAVE){
-      const int ln = idx / seg;
-      dd[base + idx] = sm[ln * ld + (idx - ln * seg)];
+    __syncthreads();
+    double q[SEG][1];
+    #pragma unroll
+    for(int e = 0; e < SEG; e++) q[e][0] = sm[lane * (SEG + 1) + e];
+    wave_tdm<SEG, 1>(q, rr, ll, vv, n);
+    #pragma unroll
+    for(int e = 0; e < SEG; e++) sm[lane * (SEG + 1) + e] = q[e][0];
+    __syncthreads();
+    if(live){
+      if(ucur){
+        #pragma unroll
+        for(int e = 0; e < SEG; e++){
+          const int idx = lane + e * TCK_WAVE;
+          if(idx < n) tck_store_halo(dd, g, A.i0 + idx, j, k, u[e] + sm[stage_pos<SEG>(idx)], false);
+        }
+      }else{
+        #pragma unroll
+        for(int e = 0; e < SEG; e++){
+          const int idx = lane + e * TCK_WAVE;
+          if(idx < n) dd[base + idx] = sm[stage_pos<SEG>(idx)];
+        }
+      }
     }
+    __syncthreads();
   }
 }
 
-/* one x line per wavefront; the line is staged through LDS so that global loads and stores
- * are contiguous across the wave (lane t touches elements t, t+64, ...), while each lane
- * solves its contiguous segment from LDS.  Segment stride m+1 avoids bank aliasing. */
+template <int SEG>
 __global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
-  __shared__ double xs_sm[TCK_BLOCK / TCK_WAVE][TCK_WAVE * (XS_MAXSEG + 1)];
-  xsolve_body(A, A.d, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE]);
+  __shared__ double xs_sm[TCK_BLOCK / TCK_WAVE][TCK_WAVE * (SEG + 1)];
+  xsolve_body<SEG>(A, A.d, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE]);
 }
 
 /* periodic lines in y (solve_in_y, :39-70) or z (solve_in_z, :72-103).  A block holds TI
@@ -360,20 +420,46 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
  * the (n-1) leading block are affine recurrences: each segment composes its rows, a fold over
  * the earlier (later) segments in LDS gives the value entering it, and the segment is replayed
  * with the reference's update formulas (tdm.c:17-50).  The Sherman-Morrison step
- * (tdm.c:92-98) then needs q0[0], q0[n-2] and q[n-1], shared through LDS. */
+ * (tdm.c:92-98) then needs q0[0], q0[n-2] and q[n-1], shared through LDS.
+ * Factor rows: below the convergence row jc they come from the tables, loaded ahead of each
+ * recurrence in groups of PS_PRE; from jc on they are the constants rc, vc (and rlast for the
+ * last row), so most segments wait on no table load at all. */
 #define PS_MAXM 32
+#define PS_PRE 8
 
 struct PSolveArgs {
   tc_geom_t g;
   double * d;
   const double * ucur;
   const double * r; const double * v; const double * q1;
-  const double * meta;  /* {den, l, u}: [3][tstride] indexed by i (y) or [3] (z) */
+  const double * meta;  /* TC_PMETA rows [7][tstride] indexed by i (y) or [7] (z) */
   long long tstride;    /* table stride between rows (tp for y, 1 for z) */
   int tcol;             /* 1: tables indexed by i as well (y) */
   int dir;              /* 1: y, 2: z */
   int i0, isz;
 };
+
+/* factor rows row0+e0 .. row0+e0+PS_PRE-1 of table t: rows below jc from the table (loaded
+ * together, ahead of use), rows from jc on the constant cval, row m-1 lval */
+__device__ __forceinline__ void ps_rows(double * out, const double * __restrict__ t, long long ts,
+    int row0, int e0, int jc, int m, double cval, double lval){
+  if(row0 + e0 < jc){
+    double tmp[PS_PRE];
+    #pragma unroll
+    for(int e = 0; e < PS_PRE; e++){
+      const int row = row0 + e0 + e;
+      tmp[e] = t[ts * (row < m ? row : m - 1)];
+    }
+    #pragma unroll
+    for(int e = 0; e < PS_PRE; e++){
+      const int row = row0 + e0 + e;
+      out[e] = row < jc ? tmp[e] : (row == m - 1 ? lval : cval);
+    }
+  }else{
+    #pragma unroll
+    for(int e = 0; e < PS_PRE; e++) out[e] = row0 + e0 + e == m - 1 ? lval : cval;
+  }
+}
 
 /* body with restrict-qualified pointers: loads of the old field and the tables may be
  * scheduled ahead of the stores to the increment array */
@@ -406,6 +492,10 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
   const double den = meta[col];
   const double tl = meta[ts + col];
   const double tu = meta[2 * ts + col];
+  const int jc = (int)meta[3 * ts + col];
+  const double rc = meta[4 * ts + col];
+  const double vc = meta[5 * ts + col];
+  const double rlast = meta[6 * ts + col];
   double * smA = ps_sm;                  /* [S][TI] */
   double * smB = ps_sm + S * TI;         /* [S][TI] */
   double * smX = ps_sm + 2 * S * TI;     /* [3][TI]: q0[0], q0[n-2], q[n-1] */
@@ -413,18 +503,24 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
   #pragma unroll
   for(int e = 0; e < PS_MAXM; e++) q[e] = (valid && e < cnt) ? d[base + step * (row0 + e)] : 0.;
   if(valid && sg == S - 1) smX[2 * TI + ti] = d[base + step * m];
+  double t[PS_PRE];
   /* forward: q0 = q0 / c0, q_i = r_i (q_i - l q_{i-1}) */
   double a = 1., b = 0.;
   #pragma unroll
-  for(int e = 0; e < PS_MAXM; e++){
-    if(e < cnt){
-      const int row = row0 + e;
-      if(row == 0){ a = 0.; b = q[e] / r[0]; }
-      else {
-        const double ri = r[ts * row];
-        const double ai = - ri * tl;
-        b = ai * b + ri * q[e];
-        a = ai * a;
+  for(int e0 = 0; e0 < PS_MAXM; e0 += PS_PRE){
+    if(e0 < cnt){
+      ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
+      #pragma unroll
+      for(int e = 0; e < PS_PRE; e++){
+        if(e0 + e < cnt){
+          const int row = row0 + e0 + e;
+          if(row == 0){ a = 0.; b = q[e0 + e] / t[e]; }
+          else {
+            const double ai = - t[e] * tl;
+            b = ai * b + t[e] * q[e0 + e];
+            a = ai * a;
+          }
+        }
       }
     }
   }
@@ -434,13 +530,19 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
   double in = 0.;
   for(int s2 = 0; s2 < sg; s2++) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
   #pragma unroll
-  for(int e = 0; e < PS_MAXM; e++){
-    if(e < cnt){
-      const int row = row0 + e;
-      if(row == 0) q[e] = q[e] / r[0];
-      else q[e] = r[ts * row] * (q[e] - tl * in);
-      in = q[e];
-      if(row == m - 1) smX[TI + ti] = q[e];
+  for(int e0 = 0; e0 < PS_MAXM; e0 += PS_PRE){
+    if(e0 < cnt){
+      ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
+      #pragma unroll
+      for(int e = 0; e < PS_PRE; e++){
+        if(e0 + e < cnt){
+          const int row = row0 + e0 + e;
+          if(row == 0) q[e0 + e] = q[e0 + e] / t[e];
+          else q[e0 + e] = t[e] * (q[e0 + e] - tl * in);
+          in = q[e0 + e];
+          if(row == m - 1) smX[TI + ti] = q[e0 + e];
+        }
+      }
     }
   }
   __syncthreads();
@@ -448,14 +550,20 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
   a = 1.;
   b = 0.;
   #pragma unroll
-  for(int e = PS_MAXM - 1; e >= 0; e--){
-    if(e < cnt){
-      const int row = row0 + e;
-      if(row == m - 1){ a = 0.; b = q[e]; }
-      else {
-        const double ai = - v[ts * row];
-        b = ai * b + q[e];
-        a = ai * a;
+  for(int e0 = PS_MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
+    if(e0 < cnt){
+      ps_rows(t, v, ts, row0, e0, jc, m, vc, 0.);
+      #pragma unroll
+      for(int e = PS_PRE - 1; e >= 0; e--){
+        if(e0 + e < cnt){
+          const int row = row0 + e0 + e;
+          if(row == m - 1){ a = 0.; b = q[e0 + e]; }
+          else {
+            const double ai = - t[e];
+            b = ai * b + q[e0 + e];
+            a = ai * a;
+          }
+        }
       }
     }
   }
@@ -465,11 +573,17 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
   in = 0.;
   for(int s2 = S - 1; s2 > sg; s2--) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
   #pragma unroll
-  for(int e = PS_MAXM - 1; e >= 0; e--){
-    if(e < cnt){
-      const int row = row0 + e;
-      if(row != m - 1) q[e] -= v[ts * row] * in;
-      in = q[e];
+  for(int e0 = PS_MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
+    if(e0 < cnt){
+      ps_rows(t, v, ts, row0, e0, jc, m, vc, 0.);
+      #pragma unroll
+      for(int e = PS_PRE - 1; e >= 0; e--){
+        if(e0 + e < cnt){
+          const int row = row0 + e0 + e;
+          if(row != m - 1) q[e0 + e] -= t[e] * in;
+          in = q[e0 + e];
+        }
+      }
     }
   }
   if(cnt > 0 && row0 == 0) smX[ti] = q[0];
@@ -478,39 +592,45 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
   const double num = smX[2 * TI + ti] - tu * smX[ti] - tl * smX[TI + ti];
   const double xl = fabs(den) < DBL_EPSILON ? 0. : num / den;
   if(!valid) return;
-  if(ucur){
-    double u[PS_MAXM];
-    #pragma unroll
-    for(int e = 0; e < PS_MAXM; e++) u[e] = e < cnt ? ucur[base + step * (row0 + e)] : 0.;
-    const double ul = sg == S - 1 ? ucur[base + step * m] : 0.;
-    #pragma unroll
-    for(int e = 0; e < PS_MAXM; e++){
-      if(e < cnt){
-        const int row = row0 + e;
-        const double x = q[e] + xl * q1[ts * row];
-        const int jj = A.dir == 1 ? 1 + row : other;
-        const int kk = A.dir == 1 ? other : 1 + row;
-        tck_store_halo(d, g, i, jj, kk, u[e] + x, false);
+  /* q1 and the old field, PS_PRE rows at a time */
+  #pragma unroll
+  for(int e0 = 0; e0 < PS_MAXM; e0 += PS_PRE){
+    if(e0 < cnt){
+      double qq[PS_PRE], uu[PS_PRE];
+      #pragma unroll
+      for(int e = 0; e < PS_PRE; e++){
+        const int row = row0 + e0 + e < m ? row0 + e0 + e : m - 1;
+        qq[e] = q1[ts * row];
+        uu[e] = ucur ? ucur[base + step * row] : 0.;
+      }
+      #pragma unroll
+      for(int e = 0; e < PS_PRE; e++){
+        if(e0 + e < cnt){
+          const int row = row0 + e0 + e;
+          const double x = q[e0 + e] + xl * qq[e];
+          if(ucur){
+            const int jj = A.dir == 1 ? 1 + row : other;
+            const int kk = A.dir == 1 ? other : 1 + row;
+            tck_store_halo(d, g, i, jj, kk, uu[e] + x, false);
+          }else{
+            d[base + step * row] = x;
+          }
+        }
       }
     }
-    if(sg == S - 1){
+  }
+  if(sg == S - 1){
+    if(ucur){
       const int jj = A.dir == 1 ? n : other;
       const int kk = A.dir == 1 ? other : n;
-      tck_store_halo(d, g, i, jj, kk, ul + xl, false);
+      tck_store_halo(d, g, i, jj, kk, ucur[base + step * m] + xl, false);
+    }else{
+      d[base + step * m] = xl;
     }
-  }else{
-    #pragma unroll
-    for(int e = 0; e < PS_MAXM; e++){
-      if(e < cnt){
-        const int row = row0 + e;
-        d[base + step * row] = q[e] + xl * q1[ts * row];
-      }
-    }
-    if(sg == S - 1) d[base + step * m] = xl;
   }
 }
 
-__global__ void __launch_bounds__(1024) tck_psolve(const PSolveArgs A){
+__global__ void __launch_bounds__(512) tck_psolve(const PSolveArgs A){
   extern __shared__ double ps_sm[];
   psolve_body(A, A.d, A.ucur, A.r, A.v, A.q1, A.meta, ps_sm);
 }
@@ -566,57 +686,97 @@ struct PXSolveArgs {
   const double * pr; const double * pv; const double * pl;
 };
 
-/* one (ky, kz) line per wavefront, staged through LDS as in tck_xsolve so that the spectrum
- * row (nx complex values, contiguous) is read and written with contiguous wave accesses */
-__global__ void __launch_bounds__(TCK_BLOCK) tck_pxsolve(const PXSolveArgs A){
-  extern __shared__ double2 px_sm[];
+/* one (ky, kz) line per wave per iteration, WPB waves per block looping over groups of WPB
+ * lines; the spectrum row and its two factor rows are staged through LDS so that every
+ * global access is contiguous across the wave (WPB keeps the staging area within 64 KB) */
+template <int SEG, int WPB>
+__global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs A){
+  __shared__ double2 smq_all[WPB][TCK_WAVE * (SEG + 1)];
+  __shared__ double smr_all[WPB][TCK_WAVE * (SEG + 1)];
+  __shared__ double smv_all[WPB][TCK_WAVE * (SEG + 1)];
   const int w = threadIdx.x / TCK_WAVE;
-  const long long wave = (long long)blockIdx.x * (blockDim.x / TCK_WAVE) + w;
   const int lane = threadIdx.x % TCK_WAVE;
-  const bool live = wave < A.nlines;
+  double2 * smq = smq_all[w];
+  double * smr = smr_all[w];
+  double * smv = smv_all[w];
   const int n = A.nx;
-  const int seg = (n + TCK_WAVE - 1) / TCK_WAVE;
-  const int ld = seg + 1;
-  const int row0 = lane * seg;
-  int cnt = n - row0;
-  cnt = cnt < 0 ? 0 : (cnt > seg ? seg : cnt);
-  const long long lw = live ? wave : 0;
-  double2 * row = A.s + lw * n;
-  const double * r = A.pr + lw * n;
-  const double * v = A.pv + lw * n;
-  double2 * sm = px_sm + (size_t)w * TCK_WAVE * ld;
-  if(live){
-    for(int idx = lane; idx < n; idx += TCK_WAVE){
-      const int ln = idx / seg;
-      sm[ln * ld + (idx - ln * seg)] = row[idx];
-    }
-  }
-  __syncthreads();
-  double q[XS_MAXSEG][2];
+  const int row0 = lane * SEG;
+  double ll[SEG];
   #pragma unroll
-  for(int e = 0; e < XS_MAXSEG; e++){
-    if(e < cnt){
-      const double2 x = sm[lane * ld + e];
+  for(int e = 0; e < SEG; e++) ll[e] = A.pl[row0 + e < n ? row0 + e : n - 1];
+  for(long long g0 = (long long)blockIdx.x * WPB; g0 < A.nlines; g0 += (long long)gridDim.x * WPB){
+    const long long line = g0 + w;
+    const bool live = line < A.nlines;
+    double2 * row = A.s + (live ? line : 0) * n;
+    const double * r = A.pr + (live ? line : 0) * n;
+    const double * v = A.pv + (live ? line : 0) * n;
+    if(live){
+      #pragma unroll
+      for(int e = 0; e < SEG; e++){
+        const int idx = lane + e * TCK_WAVE;
+        if(idx < n){
+          const int p = stage_pos<SEG>(idx);
+          smq[p] = row[idx];
+          smr[p] = r[idx];
+          smv[p] = v[idx];
+        }
+      }
+    }
+    __syncthreads();
+    double q[SEG][2], rr[SEG], vv[SEG];
+    #pragma unroll
+    for(int e = 0; e < SEG; e++){
+      const double2 x = smq[lane * (SEG + 1) + e];
       q[e][0] = x.x;
       q[e][1] = x.y;
-    }else{
-      q[e][0] = q[e][1] = 0.;
+      rr[e] = smr[lane * (SEG + 1) + e];
+      vv[e] = smv[lane * (SEG + 1) + e];
     }
-  }
-  wave_tdm<2>(q, r, v, A.pl, row0, cnt, n);
-  #pragma unroll
-  for(int e = 0; e < XS_MAXSEG; e++){
-    if(e < cnt) sm[lane * ld + e] = make_double2(q[e][0], q[e][1]);
-  }
-  __syncthreads();
-  if(!live) return;
-  for(int idx = lane; idx < n; idx += TCK_WAVE){
-    const int ln = idx / seg;
-    row[idx] = sm[ln * ld + (idx - ln * seg)];
+    wave_tdm<SEG, 2>(q, rr, ll, vv, n);
+    #pragma unroll
+    for(int e = 0; e < SEG; e++) smq[lane * (SEG + 1) + e] = make_double2(q[e][0], q[e][1]);
+    __syncthreads();
+    if(live){
+      #pragma unroll
+      for(int e = 0; e < SEG; e++){
+        const int idx = lane + e * TCK_WAVE;
+        if(idx < n) row[idx] = smq[stage_pos<SEG>(idx)];
+      }
+    }
+    __syncthreads();
   }
 }
 
 /* ------------------------------------------------------------------------------------- */
+static int seg_for(int n){
+  int seg = 1;
+  while(seg * TCK_WAVE < n) seg <<= 1;
+  return seg;
+}
+
+template <int SEG>
+static int launch_xsolve_t(const XSolveArgs & A, hipStream_t stream){
+  const long long nlines = (long long)A.g.ny * A.g.nz;
+  const int wpb = TCK_BLOCK / TCK_WAVE;
+  long long nblocks = (nlines + wpb - 1) / wpb;
+  const long long cap = (long long)num_cus() * 8;
+  if(nblocks > cap) nblocks = cap;
+  hipLaunchKernelGGL(tck_xsolve<SEG>, dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
+template <int SEG>
+static int launch_pxsolve_t(const PXSolveArgs & A, hipStream_t stream){
+  constexpr int WPB = SEG >= 16 ? 1 : 2;
+  long long nblocks = ((long long)A.nlines + WPB - 1) / WPB;
+  const long long cap = (long long)num_cus() * 16 / WPB;
+  if(nblocks > cap) nblocks = cap;
+  hipLaunchKernelGGL((tck_pxsolve<SEG, WPB>), dim3((unsigned)nblocks), dim3(WPB * TCK_WAVE), 0, stream, A);
+  TCK_CHECK_LAUNCH();
+  return 0;
+}
+
 extern "C" {
 
 /* jobs: njobs descriptors laid out as in FactorJob; fields beyond the implicit set may be null */
@@ -654,12 +814,13 @@ int tck_launch_xsolve(const tc_geom_t * g, double * d, const double * ucur, cons
   if(n > TCK_WAVE * XS_MAXSEG || n < 2) return (int)hipErrorInvalidValue;
   XSolveArgs A;
   A.g = *g; A.d = d; A.ucur = ucur; A.r = r; A.v = v; A.l = l; A.i0 = i0; A.n = n;
-  const long long nlines = (long long)g->ny * g->nz;
-  const int waves_per_block = TCK_BLOCK / TCK_WAVE;
-  const long long nblocks = (nlines + waves_per_block - 1) / waves_per_block;
-  hipLaunchKernelGGL(tck_xsolve, dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
-  TCK_CHECK_LAUNCH();
-  return 0;
+  switch(seg_for(n)){
+    case 1: return launch_xsolve_t<1>(A, stream);
+    case 2: return launch_xsolve_t<2>(A, stream);
+    case 4: return launch_xsolve_t<4>(A, stream);
+    case 8: return launch_xsolve_t<8>(A, stream);
+    default: return launch_xsolve_t<16>(A, stream);
+  }
 }
 
 int tck_launch_psolve(const tc_geom_t * g, int dir, double * d, const double * ucur,
@@ -671,7 +832,7 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, double * d, const double * u
   const int n = dir == 1 ? g->ny : g->nz;
   const int m = n - 1;
   const int S = (m + PS_MAXM - 1) / PS_MAXM;
-  if(S > 64) return (int)hipErrorInvalidValue;
+  if(S > 32) return (int)hipErrorInvalidValue;     /* n <= 1025: blocks of at most 512 threads */
   int TI = 512 / S;
   TI = TI > 64 ? 64 : (TI < 16 ? 16 : TI);
   const int nother = dir == 1 ? g->nz : g->ny;
@@ -699,15 +860,13 @@ int tck_launch_pxsolve(int nx, int nlines, double * s, const double * pr, const 
   if(nx > TCK_WAVE * XS_MAXSEG || nx < 2) return (int)hipErrorInvalidValue;
   PXSolveArgs A;
   A.nx = nx; A.nlines = nlines; A.s = (double2 *)s; A.pr = pr; A.pv = pv; A.pl = pl;
-  const int seg = (nx + TCK_WAVE - 1) / TCK_WAVE;
-  const int waves_per_block = TCK_BLOCK / TCK_WAVE;
-  const size_t lds = sizeof(double2) * (size_t)waves_per_block * TCK_WAVE * (seg + 1);
-  const long long nblocks = ((long long)nlines + waves_per_block - 1) / waves_per_block;
-  if(lds > 65536 && hipFuncSetAttribute((const void *)tck_pxsolve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-    return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(tck_pxsolve, dim3((unsigned)nblocks), dim3(TCK_BLOCK), lds, stream, A);
-  TCK_CHECK_LAUNCH();
-  return 0;
+  switch(seg_for(nx)){
+    case 1: return launch_pxsolve_t<1>(A, stream);
+    case 2: return launch_pxsolve_t<2>(A, stream);
+    case 4: return launch_pxsolve_t<4>(A, stream);
+    case 8: return launch_pxsolve_t<8>(A, stream);
+    default: return launch_pxsolve_t<16>(A, stream);
+  }
 }
 
 }
