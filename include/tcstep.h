@@ -103,6 +103,11 @@ int tc_stat_collect(tc_solver_t * s);
 int tc_stat_reset(tc_solver_t * s);
 int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num);
 
+/* no reference counterpart: launch shape of a cell kernel ("predict" or "correct"): kb planes
+ * marched per block (1..nz) and ty j rows per block of 64 x ty threads (1, 2 or 4).  Results
+ * are identical for every shape; only the speed changes */
+int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty);
+
 /* events around the kernels of one named stage ("predict", "poisson", ...) while timing */
 int tc_profile_enable(tc_solver_t * s, int on);
 int tc_profile_read(tc_solver_t * s, const char * stage, double * total_ms, long long * launches);

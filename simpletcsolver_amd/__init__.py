@@ -25,6 +25,7 @@ EXPORTED = (
     "tc_profile_enable", "tc_profile_read", "tc_device_bytes", "tc_ic_grid", "tc_ic_fields",
     "tc_arm_diagnostics", "tc_probe_copy_bandwidth", "tc_set_field_ref", "tc_get_field_ref",
     "tc_set_dt_params", "tc_stat_arm", "tc_stat_collect", "tc_stat_reset", "tc_stat_get",
+    "tc_tune_set",
 )
 
 # running statistics of statistics.c, in its accumulation order
@@ -91,6 +92,7 @@ def load_library(path=None):
     lib.tc_set_field_ref.argtypes = [vp, ctypes.c_char_p, dp]
     lib.tc_get_field_ref.argtypes = [vp, ctypes.c_char_p, dp]
     lib.tc_set_dt_params.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.tc_tune_set.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     lib.tc_stat_arm.argtypes = [vp, ctypes.c_int]
     lib.tc_stat_collect.argtypes = [vp]
     lib.tc_stat_reset.argtypes = [vp]
@@ -288,6 +290,10 @@ class Solver:
         a = np.zeros(ref_shape(name, self.nx, self.ny, self.nz))
         self._c(self._lib.tc_get_field_ref(self._h, name.encode(), _dp(a)), "tc_get_field_ref")
         return a
+
+    def tune(self, kernel, kb, ty):
+        """launch shape of a cell kernel: kb planes per block, ty j rows per block"""
+        self._c(self._lib.tc_tune_set(self._h, kernel.encode(), int(kb), int(ty)), "tc_tune_set")
 
     def stat_arm(self, on=True):
         self._c(self._lib.tc_stat_arm(self._h, int(bool(on))), "tc_stat_arm")

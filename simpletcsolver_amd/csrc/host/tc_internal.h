@@ -15,6 +15,9 @@ enum { TC_S_DT, TC_S_FACTOR, TC_S_PREDICT, TC_S_XSOLVE, TC_S_YSOLVE, TC_S_ZSOLVE
   TC_S_DIVY, TC_S_ZFFT, TC_S_PXSOLVE, TC_S_YIFFT, TC_S_CORRECT, TC_S_DIAG, TC_NSTAGE };
 
 #define TC_MAXALLOC 512
+
+/* tunable launch shapes: {planes per block, j rows per block} of the cell kernels */
+enum { TC_TUNE_PREDICT, TC_TUNE_CORRECT, TC_NTUNE };
 #define TC_MAXEV 64
 
 /* line-solve factor tables of one field (see tck_linesolve.hip) */
@@ -75,6 +78,7 @@ struct tc_solver_s {
   size_t dev_bytes;
   int prof_on;
   tc_prof_t prof[TC_NSTAGE];
+  int tune[TC_NTUNE][2];
 };
 
 int tc_fail(const char * fmt, ...);

@@ -8,14 +8,18 @@
 #include <hip/hip_runtime_api.h>
 #include "../tc_shared.h"
 
-/* tck_step.hip */
+/* tck_step.hip.  kb, ty: launch shape of the cell kernels (planes marched per block, j rows
+ * per block of 64 x ty threads; ty in {1, 2, 4}).  dtmin: TC_DTSLOTS accumulators, one
+ * every TC_DTSTRIDE entries */
+#define TC_DTSLOTS 64
+#define TC_DTSTRIDE 16
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     const double * const cur[4], const double * p, double * const nxt[4],
     const double * const srcb[4], double * const srca[4], const double * dt, int rkstep,
-    int final_here, hipStream_t stream);
+    int final_here, int kb, int ty, hipStream_t stream);
 int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     double * ux, double * uy, double * uz, double * p, const double * psi, const double * dt,
-    int rkstep, int mode, unsigned long long * dtmin, hipStream_t stream);
+    int rkstep, int mode, unsigned long long * dtmin, int kb, int ty, hipStream_t stream);
 int tck_launch_dt(const tc_geom_t * g, const tc_coef_t * c, const double * ux, const double * uy,
     const double * uz, unsigned long long * dtmin, hipStream_t stream);
 int tck_launch_dt_finalise(unsigned long long * dtmin, double * dt, double coef_adv,

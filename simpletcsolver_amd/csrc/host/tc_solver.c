@@ -85,15 +85,6 @@ static int dalloc(tc_solver_t * s, double ** p, size_t n){
  * metrics and per-i coefficient tables (domain.c:259-357, predict/*.c init_laplacians,
  * compute_potential.c:173-211), evaluated with the operand order of the reference
  * ------------------------------------------------------------------------------------- */
-enum {
-  T_HXXF, T_HXXC, T_HYXF, T_HYXC, T_JDXF, T_JDXC,
-  T_AXF, T_IJDXF, T_IHXF, T_IHYF, T_JHXF, T_JHYF, T_JHZF, T_L1F, T_NUJF, T_N2JF, T_D1M, T_D1P,
-  T_AYC, T_AZC, T_IJDXC, T_IHYC, T_JHYC, T_JHZC, T_L1C, T_HDC, T_NUJC, T_N2JC, T_DAM, T_DAP,
-  T_LXU_L, T_LXU_C, T_LXU_U, T_LYU_L, T_LYU_C, T_LYU_U,
-  T_LXC_L, T_LXC_C, T_LXC_U, T_LYC_L, T_LYC_C, T_LYC_U,
-  T_PLX_L, T_PLX_U, T_PLY, T_PLZ,
-  T_NTAB
-};
 
 static int build_tables(tc_solver_t * s, const double * xf_in, const double * xc_in){
   const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
@@ -251,6 +242,8 @@ static int build_tables(tc_solver_t * s, const double * xf_in, const double * xc
   c->lxc_l = DT(T_LXC_L); c->lxc_c = DT(T_LXC_C); c->lxc_u = DT(T_LXC_U);
   c->lyc_l = DT(T_LYC_L); c->lyc_c = DT(T_LYC_C); c->lyc_u = DT(T_LYC_U);
   c->plx_l = DT(T_PLX_L); c->plx_u = DT(T_PLX_U); c->ply = DT(T_PLY); c->plz = DT(T_PLZ);
+  c->tab = s->d_tab;
+  c->tp = tp;
   #undef DT
   /* predict/ux.c:70-78 */
   c->lapz_l = 1. / hz / hz;
@@ -348,6 +341,8 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
   tc_solver_t * s = calloc(1, sizeof(tc_solver_t));
   if(!s) return tc_fail("host allocation failed");
   s->p = *p;
+  s->tune[TC_TUNE_PREDICT][0] = 16; s->tune[TC_TUNE_PREDICT][1] = 4;
+  s->tune[TC_TUNE_CORRECT][0] = 8; s->tune[TC_TUNE_CORRECT][1] = 4;
   if(hipSetDevice(p->device) != hipSuccess){
     free(s);
     return tc_fail("hipSetDevice(%d) failed", p->device);
@@ -380,7 +375,7 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
   if(!rc) rc |= dalloc(s, &s->d_p, n3);
   if(!rc) rc |= dalloc(s, &s->d_psi, n3);
   if(!rc) rc |= dalloc(s, &s->d_dt, 4);
-  if(!rc) rc |= dalloc(s, (double **)&s->d_dtmin, 2);
+  if(!rc) rc |= dalloc(s, (double **)&s->d_dtmin, (size_t)TC_DTSLOTS * TC_DTSTRIDE);
   if(!rc){
     const long long nb = tck_diag_cells_blocks(g) > tck_diag_walls_blocks(g) ? tck_diag_cells_blocks(g) : tck_diag_walls_blocks(g);
     rc |= dalloc(s, &s->d_part, (size_t)nb * 8);
@@ -721,7 +716,7 @@ static int predict(tc_solver_t * s, int rkstep){
     srca[f] = rkstep < 2 ? s->src[f][rkstep & 1] : NULL;
   }
   PROF(TC_S_PREDICT, tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, srcb, srca,
-      s->d_dt, rkstep, !any_implicit, s->stream));
+      s->d_dt, rkstep, !any_implicit, s->tune[TC_TUNE_PREDICT][0], s->tune[TC_TUNE_PREDICT][1], s->stream));
   if(any_implicit){
     int last = 0;
     for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
@@ -762,7 +757,7 @@ static int potential(tc_solver_t * s, int rkstep){
 static int correct(tc_solver_t * s, int rkstep, int mode, int want_cfl){
   PROF(TC_S_CORRECT, tck_launch_correct(&s->g, &s->coef, &s->flags, tc_field_ptr(s, TC_FUX),
       tc_field_ptr(s, TC_FUY), tc_field_ptr(s, TC_FUZ), s->d_p, s->d_psi, s->d_dt, rkstep, mode,
-      want_cfl ? s->d_dtmin : NULL, s->stream));
+      want_cfl ? s->d_dtmin : NULL, s->tune[TC_TUNE_CORRECT][0], s->tune[TC_TUNE_CORRECT][1], s->stream));
   return 0;
 }
 
@@ -921,6 +916,19 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
   if(logging_values(s, v)) return 1;
   out[0] = s->nu * v[1];
   out[1] = s->kappa * v[2];
+  return 0;
+}
+
+/* launch shapes of the cell kernels ("predict", "correct"): planes per block and j rows per
+ * block; results do not depend on them */
+int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
+  int n = -1;
+  if(kernel && strcmp(kernel, "predict") == 0) n = TC_TUNE_PREDICT;
+  if(kernel && strcmp(kernel, "correct") == 0) n = TC_TUNE_CORRECT;
+  if(n < 0) return tc_fail("tc_tune_set: unknown kernel '%s'", kernel ? kernel : "(null)");
+  if(kb < 1 || kb > s->p.nz || !(ty == 1 || ty == 2 || ty == 4)) return tc_fail("tc_tune_set: kb %d, ty %d out of range", kb, ty);
+  s->tune[n][0] = kb;
+  s->tune[n][1] = ty;
   return 0;
 }
 

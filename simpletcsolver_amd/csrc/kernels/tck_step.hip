@@ -23,6 +23,34 @@
 __constant__ double c_rka[3] = {32. / 60., 25. / 60., 45. / 60.};
 __constant__ double c_rkb[3] = { 0. / 60., -17. / 60., -25. / 60.};
 
+/* Launch shape of the cell kernels: a block of 64 x ty threads covers an (i, j) tile and
+ * marches kb planes in k, so the k-1 and k+1 planes a thread reads were read by the same block
+ * one iteration earlier.  Blocks are numbered tile-major (i tile fastest, then j, then the k
+ * chunk) and renumbered so that each of the 8 XCDs (which receive launch-order block ids
+ * round-robin and keep their own L2) works through one contiguous eighth of the tiles: the
+ * halo rows and planes a block shares with its neighbours then sit in the same L2. */
+#define TCK_NXCD 8
+#define TCK_DTSLOTS 64      /* CFL minimum accumulators, one 128 B line apart */
+#define TCK_DTSTRIDE 16
+
+struct TileMap {
+  int nbx, nby, nbz;   /* tiles in i, j and k chunks */
+  int kb;              /* planes per block */
+  long long total;
+};
+
+__device__ __forceinline__ void tile_of(const TileMap & T, int * bx, int * by, int * bz){
+  long long t = blockIdx.x;
+  if(T.total % TCK_NXCD == 0){
+    const long long per = T.total / TCK_NXCD;
+    t = (t % TCK_NXCD) * per + t / TCK_NXCD;
+  }
+  *bx = (int)(t % T.nbx);
+  t /= T.nbx;
+  *by = (int)(t % T.nby);
+  *bz = (int)(t / T.nby);
+}
+
 struct PredictArgs {
   tc_geom_t g;
   tc_coef_t c;
@@ -34,7 +62,16 @@ struct PredictArgs {
   const double * dt;
   int rkstep;
   int final_here;   /* 1: no line solve follows, write u + delta with halos */
+  TileMap tm;
 };
+
+/* coefficient row t at index ii, from the block's LDS copy of rows 0..T_NPRED-1 over the
+ * window ib .. ib + TCK_TW - 1 */
+#define TCK_TW (TCK_WAVE + 2)
+/* keeps the loads of one field's section from being hoisted above the previous section's
+ * stores: shorter live ranges, fewer registers, higher occupancy */
+#define TCK_SECTION_FENCE() asm volatile("" ::: "memory")
+#define CT(t, ii) (smt[(t) * TCK_TW + (ii) - ib])
 
 /* body with restrict-qualified pointers, so that the loads of each field section are free
  * to move ahead of the stores of the previous one */
@@ -44,16 +81,16 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     double * __restrict__ nxo, double * __restrict__ nyo, double * __restrict__ nzo, double * __restrict__ nto,
     const double * __restrict__ sbx, const double * __restrict__ sby, const double * __restrict__ sbz,
     const double * __restrict__ sbt,
-    double * __restrict__ sax, double * __restrict__ say, double * __restrict__ saz, double * __restrict__ sat){
+    double * __restrict__ sax, double * __restrict__ say, double * __restrict__ saz, double * __restrict__ sat,
+    const int i, const int j, const int k, const long long kb0, const double * smt, const int ib){
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
   if(i > g.nx || j > g.ny) return;
-  const long long o = tck_idx(g, i, j, k);
-  const long long sy = g.px;
-  const long long sz = g.plane;
+  /* the field pointers start at plane k-1 (kb0): offsets inside the 3-plane window fit in
+   * 32 bits, so every load is a base + 32-bit offset */
+  const int sy = g.px;
+  const int sz = (int)g.plane;
+  const int o = (i + TC_XOFF) + sy * j + sz;
   const double nu = c.nu;
   const double nu2 = 2. * nu;
   const double dt = *A.dt;
@@ -68,77 +105,78 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     double a = 0., gg = 0.;
     const double u0 = ux[o], uxm = ux[o - 1], uxp = ux[o + 1];
     { /* advection_x :92-124 */
-      const double ux_xm = c.axf[i - 1] * uxm + c.axf[i] * u0;
-      const double ux_xp = c.axf[i] * u0 + c.axf[i + 1] * uxp;
+      const double ux_xm = CT(T_AXF, i - 1) * uxm + CT(T_AXF, i) * u0;
+      const double ux_xp = CT(T_AXF, i) * u0 + CT(T_AXF, i + 1) * uxp;
       const double l = - 0.5 * ux_xm;
       const double u = + 0.5 * ux_xp;
       const double cc = - l - u;
-      a -= c.ijdxf[i] * (+ l * uxm + cc * u0 + u * uxp);
+      a -= CT(T_IJDXF, i) * (+ l * uxm + cc * u0 + u * uxp);
     }
     { /* advection_y :126-159 */
-      const double uy_ym = c.ayc[i - 1] * uy[o - 1] + c.ayc[i] * uy[o];
-      const double uy_yp = c.ayc[i - 1] * uy[o - 1 + sy] + c.ayc[i] * uy[o + sy];
+      const double uy_ym = CT(T_AYC, i - 1) * uy[o - 1] + CT(T_AYC, i) * uy[o];
+      const double uy_yp = CT(T_AYC, i - 1) * uy[o - 1 + sy] + CT(T_AYC, i) * uy[o + sy];
       const double l = - 0.5 * uy_ym;
       const double u = + 0.5 * uy_yp;
       const double cc = - l - u;
-      a -= c.ijdxf[i] * (+ l * ux[o - sy] + cc * u0 + u * ux[o + sy]);
+      a -= CT(T_IJDXF, i) * (+ l * ux[o - sy] + cc * u0 + u * ux[o + sy]);
     }
     { /* advection_z :161-192 */
-      const double uz_zm = c.azc[i - 1] * uz[o - 1] + c.azc[i] * uz[o];
-      const double uz_zp = c.azc[i - 1] * uz[o - 1 + sz] + c.azc[i] * uz[o + sz];
+      const double uz_zm = CT(T_AZC, i - 1) * uz[o - 1] + CT(T_AZC, i) * uz[o];
+      const double uz_zp = CT(T_AZC, i - 1) * uz[o - 1 + sz] + CT(T_AZC, i) * uz[o + sz];
       const double l = - 0.5 * uz_zm;
       const double u = + 0.5 * uz_zp;
       const double cc = - l - u;
-      a -= c.ijdxf[i] * (+ l * ux[o - sz] + cc * u0 + u * ux[o + sz]);
+      a -= CT(T_IJDXF, i) * (+ l * ux[o - sz] + cc * u0 + u * ux[o + sz]);
     }
     { /* advection_1 :194-226 (centrifugal) */
       const double uy_xm = + 0.5 * uy[o - 1] + 0.5 * uy[o - 1 + sy];
       const double uy_xp = + 0.5 * uy[o] + 0.5 * uy[o + sy];
-      a += c.ijdxf[i] * (+ c.d1m[i] * uy_xm * uy_xm + c.d1p[i] * uy_xp * uy_xp);
+      a += CT(T_IJDXF, i) * (+ CT(T_D1M, i) * uy_xm * uy_xm + CT(T_D1P, i) * uy_xp * uy_xp);
     }
     { /* diffusion_x :228-248 */
-      const double v = nu2 * (+ c.lxu_l[i] * uxm + c.lxu_c[i] * u0 + c.lxu_u[i] * uxp);
+      const double v = nu2 * (+ CT(T_LXU_L, i) * uxm + CT(T_LXU_C, i) * u0 + CT(T_LXU_U, i) * uxp);
       if(ix) gg += v; else a += v;
     }
     { /* diffusion_y0 :250-269 */
-      const double v = nu * (+ c.lyu_l[i] * ux[o - sy] + c.lyu_c[i] * u0 + c.lyu_u[i] * ux[o + sy]);
+      const double v = nu * (+ CT(T_LYU_L, i) * ux[o - sy] + CT(T_LYU_C, i) * u0 + CT(T_LYU_U, i) * ux[o + sy]);
       if(iy) gg += v; else a += v;
     }
     { /* diffusion_y1 :271-297; lyx1 (lyx.c:56-95) and lxy (lxy.c:19-49) at j and j+1 */
-      const double lyx_ym = c.l1f[i] * (+ 0.5 * uy[o - 1] + 0.5 * uy[o]);
-      const double lyx_yp = c.l1f[i] * (+ 0.5 * uy[o - 1 + sy] + 0.5 * uy[o + sy]);
-      const double lxy_ym = c.ihxf[i] * (- uy[o - 1] + uy[o]);
-      const double lxy_yp = c.ihxf[i] * (- uy[o - 1 + sy] + uy[o + sy]);
-      a += c.nujf[i] * (- c.jhyf[i] * (lyx_ym + lxy_ym) + c.jhyf[i] * (lyx_yp + lxy_yp));
+      const double lyx_ym = CT(T_L1F, i) * (+ 0.5 * uy[o - 1] + 0.5 * uy[o]);
+      const double lyx_yp = CT(T_L1F, i) * (+ 0.5 * uy[o - 1 + sy] + 0.5 * uy[o + sy]);
+      const double lxy_ym = CT(T_IHXF, i) * (- uy[o - 1] + uy[o]);
+      const double lxy_yp = CT(T_IHXF, i) * (- uy[o - 1 + sy] + uy[o + sy]);
+      a += CT(T_NUJF, i) * (- CT(T_JHYF, i) * (lyx_ym + lxy_ym) + CT(T_JHYF, i) * (lyx_yp + lxy_yp));
     }
     { /* diffusion_z0 :299-318 */
       const double v = nu * (+ c.lapz_l * ux[o - sz] + c.lapz_c * u0 + c.lapz_u * ux[o + sz]);
       if(iz) gg += v; else a += v;
     }
     { /* diffusion_z1 :320-342; lxz (lxz.c:10-44) at k and k+1 */
-      const double lxz_zm = c.ihxf[i] * (- uz[o - 1] + uz[o]);
-      const double lxz_zp = c.ihxf[i] * (- uz[o - 1 + sz] + uz[o + sz]);
-      a += c.nujf[i] * (- c.jhzf[i] * lxz_zm + c.jhzf[i] * lxz_zp);
+      const double lxz_zm = CT(T_IHXF, i) * (- uz[o - 1] + uz[o]);
+      const double lxz_zp = CT(T_IHXF, i) * (- uz[o - 1 + sz] + uz[o + sz]);
+      a += CT(T_NUJF, i) * (- CT(T_JHZF, i) * lxz_zm + CT(T_JHZF, i) * lxz_zp);
     }
     { /* diffusion_a :344-375; lyy0 (lyy.c:24-54) + lyy1 (lyy.c:56-95) at i-1 and i */
-      const double lyy0_m = c.ihyc[i - 1] * (- uy[o - 1] + uy[o - 1 + sy]);
-      const double lyy1_m = c.l1c[i - 1] * (+ 0.5 * uxm + 0.5 * u0);
-      const double lyy0_p = c.ihyc[i] * (- uy[o] + uy[o + sy]);
-      const double lyy1_p = c.l1c[i] * (+ 0.5 * u0 + 0.5 * uxp);
+      const double lyy0_m = CT(T_IHYC, i - 1) * (- uy[o - 1] + uy[o - 1 + sy]);
+      const double lyy1_m = CT(T_L1C, i - 1) * (+ 0.5 * uxm + 0.5 * u0);
+      const double lyy0_p = CT(T_IHYC, i) * (- uy[o] + uy[o + sy]);
+      const double lyy1_p = CT(T_L1C, i) * (+ 0.5 * u0 + 0.5 * uxp);
       const double lyy_xm = lyy0_m + lyy1_m;
       const double lyy_xp = lyy0_p + lyy1_p;
-      a -= c.n2jf[i] * (+ c.d1m[i] * lyy_xm + c.d1p[i] * lyy_xp);
+      a -= CT(T_N2JF, i) * (+ CT(T_D1M, i) * lyy_xm + CT(T_D1P, i) * lyy_xp);
     }
     /* pressure :377-394 */
-    gg -= c.ihxf[i] * (- p[o - 1] + p[o]);
+    gg -= CT(T_IHXF, i) * (- p[o - 1] + p[o]);
     double b = 0.;
     if(sbx) b = sbx[o];
     const double delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sax) sax[o] = a;
-    if(A.final_here) tck_store_halo(nxo, g, i, j, k, u0 + delta, false);
+    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, u0 + delta, false);
     else nxo[o] = delta;
   }
 
+  TCK_SECTION_FENCE();
   /* ---------------- uy, centres i = 1..nx (predict/uy.c) ---------------- */
   {
     double a = 0., gg = 0.;
@@ -148,83 +186,84 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     /* ux at (i, j-1), (i, j), (i+1, j-1), (i+1, j) */
     const double x00 = ux[o - sy], x01 = ux[o], x10 = ux[o + 1 - sy], x11 = ux[o + 1];
     { /* advection_x :93-126 */
-      const double ux_xm = c.axf[i] * x00 + c.axf[i] * x01;
-      const double ux_xp = c.axf[i + 1] * x10 + c.axf[i + 1] * x11;
+      const double ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
+      const double ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
       const double l = - 0.5 * ux_xm;
       const double u = + 0.5 * ux_xp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * vxm + cc * v0 + u * vxp);
+      a -= CT(T_IJDXC, i) * (+ l * vxm + cc * v0 + u * vxp);
     }
     { /* advection_y :128-156 */
-      const double uy_ym = c.ayc[i] * vym + c.ayc[i] * v0;
-      const double uy_yp = c.ayc[i] * v0 + c.ayc[i] * vyp;
+      const double uy_ym = CT(T_AYC, i) * vym + CT(T_AYC, i) * v0;
+      const double uy_yp = CT(T_AYC, i) * v0 + CT(T_AYC, i) * vyp;
       const double l = - 0.5 * uy_ym;
       const double u = + 0.5 * uy_yp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * vym + cc * v0 + u * vyp);
+      a -= CT(T_IJDXC, i) * (+ l * vym + cc * v0 + u * vyp);
     }
     { /* advection_z :158-186 */
-      const double uz_zm = c.azc[i] * uz[o - sy] + c.azc[i] * uz[o];
-      const double uz_zp = c.azc[i] * uz[o - sy + sz] + c.azc[i] * uz[o + sz];
+      const double uz_zm = CT(T_AZC, i) * uz[o - sy] + CT(T_AZC, i) * uz[o];
+      const double uz_zp = CT(T_AZC, i) * uz[o - sy + sz] + CT(T_AZC, i) * uz[o + sz];
       const double l = - 0.5 * uz_zm;
       const double u = + 0.5 * uz_zp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * vzm + cc * v0 + u * vzp);
+      a -= CT(T_IJDXC, i) * (+ l * vzm + cc * v0 + u * vzp);
     }
     { /* advection_1 :188-223 (Coriolis) */
       const double uy_ym = + 0.5 * vym + 0.5 * v0;
       const double uy_yp = + 0.5 * v0 + 0.5 * vyp;
       const double ux_ym = + 0.5 * x00 + 0.5 * x10;
       const double ux_yp = + 0.5 * x01 + 0.5 * x11;
-      a -= c.ijdxc[i] * (+ c.hdc[i] * uy_ym * ux_ym + c.hdc[i] * uy_yp * ux_yp);
+      a -= CT(T_IJDXC, i) * (+ CT(T_HDC, i) * uy_ym * ux_ym + CT(T_HDC, i) * uy_yp * ux_yp);
     }
     { /* diffusion_x0 :225-244 */
-      const double v = nu * (+ c.lxc_l[i] * vxm + c.lxc_c[i] * v0 + c.lxc_u[i] * vxp);
+      const double v = nu * (+ CT(T_LXC_L, i) * vxm + CT(T_LXC_C, i) * v0 + CT(T_LXC_U, i) * vxp);
       if(ix) gg += v; else a += v;
     }
     /* lyx0 (lyx.c:24-54), lyx1 (:56-95) and lxy (lxy.c:19-49) at faces i and i+1 */
-    const double lyx0_m = c.ihyf[i] * (- x00 + x01);
-    const double lyx0_p = c.ihyf[i + 1] * (- x10 + x11);
-    const double lyx1_m = c.l1f[i] * (+ (1 == i ? 0. : 0.5) * vxm + 0.5 * v0);
-    const double lyx1_p = c.l1f[i + 1] * (+ 0.5 * v0 + (g.nx == i ? 0. : 0.5) * vxp);
+    const double lyx0_m = CT(T_IHYF, i) * (- x00 + x01);
+    const double lyx0_p = CT(T_IHYF, i + 1) * (- x10 + x11);
+    const double lyx1_m = CT(T_L1F, i) * (+ (1 == i ? 0. : 0.5) * vxm + 0.5 * v0);
+    const double lyx1_p = CT(T_L1F, i + 1) * (+ 0.5 * v0 + (g.nx == i ? 0. : 0.5) * vxp);
     const double lyx_xm = lyx0_m + lyx1_m;
     const double lyx_xp = lyx0_p + lyx1_p;
     { /* diffusion_x1 :246-274 */
-      a += c.nujc[i] * (- c.jhxf[i] * lyx_xm + c.jhxf[i + 1] * lyx_xp);
+      a += CT(T_NUJC, i) * (- CT(T_JHXF, i) * lyx_xm + CT(T_JHXF, i + 1) * lyx_xp);
     }
     { /* diffusion_y0 :276-296 */
-      const double v = nu2 * (+ c.lyc_l[i] * vym + c.lyc_c[i] * v0 + c.lyc_u[i] * vyp);
+      const double v = nu2 * (+ CT(T_LYC_L, i) * vym + CT(T_LYC_C, i) * v0 + CT(T_LYC_U, i) * vyp);
       if(iy) gg += v; else a += v;
     }
     { /* diffusion_y1 :298-320; lyy1 (lyy.c:56-95) at j-1 and j */
-      const double lyy1_m = c.l1c[i] * (+ 0.5 * x00 + 0.5 * x10);
-      const double lyy1_p = c.l1c[i] * (+ 0.5 * x01 + 0.5 * x11);
-      a += c.n2jc[i] * (- c.jhyc[i] * lyy1_m + c.jhyc[i] * lyy1_p);
+      const double lyy1_m = CT(T_L1C, i) * (+ 0.5 * x00 + 0.5 * x10);
+      const double lyy1_p = CT(T_L1C, i) * (+ 0.5 * x01 + 0.5 * x11);
+      a += CT(T_N2JC, i) * (- CT(T_JHYC, i) * lyy1_m + CT(T_JHYC, i) * lyy1_p);
     }
     { /* diffusion_z0 :322-341 */
       const double v = nu * (+ c.lapz_l * vzm + c.lapz_c * v0 + c.lapz_u * vzp);
       if(iz) gg += v; else a += v;
     }
     { /* diffusion_z1 :343-365; lyz (lyz.c:10-44) at k and k+1 */
-      const double lyz_zm = c.ihyc[i] * (- uz[o - sy] + uz[o]);
-      const double lyz_zp = c.ihyc[i] * (- uz[o - sy + sz] + uz[o + sz]);
-      a += c.nujc[i] * (- c.jhzc[i] * lyz_zm + c.jhzc[i] * lyz_zp);
+      const double lyz_zm = CT(T_IHYC, i) * (- uz[o - sy] + uz[o]);
+      const double lyz_zp = CT(T_IHYC, i) * (- uz[o - sy + sz] + uz[o + sz]);
+      a += CT(T_NUJC, i) * (- CT(T_JHZC, i) * lyz_zm + CT(T_JHZC, i) * lyz_zp);
     }
     { /* diffusion_a :367-400 */
-      const double lxy_xm = c.ihxf[i] * (- vxm + v0);
-      const double lxy_xp = c.ihxf[i + 1] * (- v0 + vxp);
-      a += c.nujc[i] * (+ c.dam[i] * (lyx_xm + lxy_xm) + c.dap[i] * (lyx_xp + lxy_xp));
+      const double lxy_xm = CT(T_IHXF, i) * (- vxm + v0);
+      const double lxy_xp = CT(T_IHXF, i + 1) * (- v0 + vxp);
+      a += CT(T_NUJC, i) * (+ CT(T_DAM, i) * (lyx_xm + lxy_xm) + CT(T_DAP, i) * (lyx_xp + lxy_xp));
     }
     /* pressure :402-419 */
-    gg -= c.ihyc[i] * (- p[o - sy] + p[o]);
+    gg -= CT(T_IHYC, i) * (- p[o - sy] + p[o]);
     double b = 0.;
     if(sby) b = sby[o];
     const double delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(say) say[o] = a;
-    if(A.final_here) tck_store_halo(nyo, g, i, j, k, v0 + delta, false);
+    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, v0 + delta, false);
     else nyo[o] = delta;
   }
 
+  TCK_SECTION_FENCE();
   /* ---------------- uz, centres (predict/uz.c) ---------------- */
   {
     double a = 0., gg = 0.;
@@ -236,46 +275,46 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     /* uy at (j, k-1), (j, k), (j+1, k-1), (j+1, k) */
     const double y00 = uy[o - sz], y01 = uy[o], y10 = uy[o + sy - sz], y11 = uy[o + sy];
     { /* advection_x :88-121 */
-      const double ux_xm = c.axf[i] * x00 + c.axf[i] * x01;
-      const double ux_xp = c.axf[i + 1] * x10 + c.axf[i + 1] * x11;
+      const double ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
+      const double ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
       const double l = - 0.5 * ux_xm;
       const double u = + 0.5 * ux_xp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * wxm + cc * w0 + u * wxp);
+      a -= CT(T_IJDXC, i) * (+ l * wxm + cc * w0 + u * wxp);
     }
     { /* advection_y :123-152 */
-      const double uy_ym = c.ayc[i] * y00 + c.ayc[i] * y01;
-      const double uy_yp = c.ayc[i] * y10 + c.ayc[i] * y11;
+      const double uy_ym = CT(T_AYC, i) * y00 + CT(T_AYC, i) * y01;
+      const double uy_yp = CT(T_AYC, i) * y10 + CT(T_AYC, i) * y11;
       const double l = - 0.5 * uy_ym;
       const double u = + 0.5 * uy_yp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * wym + cc * w0 + u * wyp);
+      a -= CT(T_IJDXC, i) * (+ l * wym + cc * w0 + u * wyp);
     }
     { /* advection_z :154-181 */
-      const double uz_zm = c.azc[i] * wzm + c.azc[i] * w0;
-      const double uz_zp = c.azc[i] * w0 + c.azc[i] * wzp;
+      const double uz_zm = CT(T_AZC, i) * wzm + CT(T_AZC, i) * w0;
+      const double uz_zp = CT(T_AZC, i) * w0 + CT(T_AZC, i) * wzp;
       const double l = - 0.5 * uz_zm;
       const double u = + 0.5 * uz_zp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * wzm + cc * w0 + u * wzp);
+      a -= CT(T_IJDXC, i) * (+ l * wzm + cc * w0 + u * wzp);
     }
     { /* diffusion_x0 :183-202 */
-      const double v = nu * (+ c.lxc_l[i] * wxm + c.lxc_c[i] * w0 + c.lxc_u[i] * wxp);
+      const double v = nu * (+ CT(T_LXC_L, i) * wxm + CT(T_LXC_C, i) * w0 + CT(T_LXC_U, i) * wxp);
       if(ix) gg += v; else a += v;
     }
     { /* diffusion_x1 :204-231; lzx (lzx.c:9-42) at faces i and i+1 */
       const double lzx_xm = c.ihz * (- x00 + x01);
       const double lzx_xp = c.ihz * (- x10 + x11);
-      a += c.nujc[i] * (- c.jhxf[i] * lzx_xm + c.jhxf[i + 1] * lzx_xp);
+      a += CT(T_NUJC, i) * (- CT(T_JHXF, i) * lzx_xm + CT(T_JHXF, i + 1) * lzx_xp);
     }
     { /* diffusion_y0 :233-252 */
-      const double v = nu * (+ c.lyc_l[i] * wym + c.lyc_c[i] * w0 + c.lyc_u[i] * wyp);
+      const double v = nu * (+ CT(T_LYC_L, i) * wym + CT(T_LYC_C, i) * w0 + CT(T_LYC_U, i) * wyp);
       if(iy) gg += v; else a += v;
     }
     { /* diffusion_y1 :254-277; lzy (lzy.c:9-42) at j and j+1 */
       const double lzy_ym = c.ihz * (- y00 + y01);
       const double lzy_yp = c.ihz * (- y10 + y11);
-      a += c.nujc[i] * (- c.jhyc[i] * lzy_ym + c.jhyc[i] * lzy_yp);
+      a += CT(T_NUJC, i) * (- CT(T_JHYC, i) * lzy_ym + CT(T_JHYC, i) * lzy_yp);
     }
     { /* diffusion_z :279-299 */
       const double v = nu2 * (+ c.lapz_l * wzm + c.lapz_c * w0 + c.lapz_u * wzp);
@@ -287,10 +326,11 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbz) b = sbz[o];
     const double delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(saz) saz[o] = a;
-    if(A.final_here) tck_store_halo(nzo, g, i, j, k, w0 + delta, false);
+    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, w0 + delta, false);
     else nzo[o] = delta;
   }
 
+  TCK_SECTION_FENCE();
   /* ---------------- T, centres (predict/t.c) ---------------- */
   if(A.f.scalar){
     const double kappa = c.kappa;
@@ -299,33 +339,33 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     const double tym = t[o - sy], typ = t[o + sy];
     const double tzm = t[o - sz], tzp = t[o + sz];
     { /* advection_x :88-119 */
-      const double ux_xm = c.jhxf[i] * ux[o];
-      const double ux_xp = c.jhxf[i + 1] * ux[o + 1];
+      const double ux_xm = CT(T_JHXF, i) * ux[o];
+      const double ux_xp = CT(T_JHXF, i + 1) * ux[o + 1];
       const double l = - 0.5 * ux_xm;
       const double u = + 0.5 * ux_xp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * txm + cc * t0 + u * txp);
+      a -= CT(T_IJDXC, i) * (+ l * txm + cc * t0 + u * txp);
     }
     { /* advection_y :121-148 */
-      const double uy_ym = c.jhyc[i] * uy[o];
-      const double uy_yp = c.jhyc[i] * uy[o + sy];
+      const double uy_ym = CT(T_JHYC, i) * uy[o];
+      const double uy_yp = CT(T_JHYC, i) * uy[o + sy];
       const double l = - 0.5 * uy_ym;
       const double u = + 0.5 * uy_yp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * tym + cc * t0 + u * typ);
+      a -= CT(T_IJDXC, i) * (+ l * tym + cc * t0 + u * typ);
     }
     { /* advection_z :150-176 */
-      const double uz_zm = c.jhzc[i] * uz[o];
-      const double uz_zp = c.jhzc[i] * uz[o + sz];
+      const double uz_zm = CT(T_JHZC, i) * uz[o];
+      const double uz_zp = CT(T_JHZC, i) * uz[o + sz];
       const double l = - 0.5 * uz_zm;
       const double u = + 0.5 * uz_zp;
       const double cc = - l - u;
-      a -= c.ijdxc[i] * (+ l * tzm + cc * t0 + u * tzp);
+      a -= CT(T_IJDXC, i) * (+ l * tzm + cc * t0 + u * tzp);
     }
     { /* diffusion_x/y/z :178-239 */
-      const double vx = kappa * (+ c.lxc_l[i] * txm + c.lxc_c[i] * t0 + c.lxc_u[i] * txp);
+      const double vx = kappa * (+ CT(T_LXC_L, i) * txm + CT(T_LXC_C, i) * t0 + CT(T_LXC_U, i) * txp);
       if(ix) gg += vx; else a += vx;
-      const double vy = kappa * (+ c.lyc_l[i] * tym + c.lyc_c[i] * t0 + c.lyc_u[i] * typ);
+      const double vy = kappa * (+ CT(T_LYC_L, i) * tym + CT(T_LYC_C, i) * t0 + CT(T_LYC_U, i) * typ);
       if(iy) gg += vy; else a += vy;
       const double vz = kappa * (+ c.lapz_l * tzm + c.lapz_c * t0 + c.lapz_u * tzp);
       if(iz) gg += vz; else a += vz;
@@ -334,14 +374,37 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbt) b = sbt[o];
     const double delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sat) sat[o] = a;
-    if(A.final_here) tck_store_halo(nto, g, i, j, k, t0 + delta, false);
+    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, t0 + delta, false);
     else nto[o] = delta;
   }
 }
 
-__global__ void __launch_bounds__(TCK_BLOCK) tck_predict(const PredictArgs A){
-  predict_body(A, A.ux, A.uy, A.uz, A.t, A.p, A.nx_, A.ny_, A.nz_, A.nt, A.sbx, A.sby, A.sbz, A.sbt,
-      A.sax, A.say, A.saz, A.sat);
+__global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) tck_predict(const PredictArgs A){
+  __shared__ double smt[T_NPRED * TCK_TW];
+  int bx, by, bz;
+  tile_of(A.tm, &bx, &by, &bz);
+  const int ib = bx * TCK_WAVE;           /* i of LDS column 0: the tile starts at i = ib + 1 */
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int nth = blockDim.x * blockDim.y;
+  for(int q = tid; q < T_NPRED * TCK_TW; q += nth){
+    const int t = q / TCK_TW;
+    const int ii = ib + q % TCK_TW;
+    smt[q] = ii < A.c.tp ? A.c.tab[(long long)t * A.c.tp + ii] : 0.;
+  }
+  __syncthreads();
+  const int i = 1 + ib + threadIdx.x;
+  const int j = 1 + by * blockDim.y + threadIdx.y;
+  const int k0 = 1 + bz * A.tm.kb;
+  const int k1 = min(k0 + A.tm.kb, A.g.nz + 1);
+  for(int k = k0; k < k1; k++){
+    const long long kb0 = (long long)(k - 1) * A.g.plane;
+    #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
+    predict_body(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, RB(A.t), A.p + kb0,
+        A.nx_ + kb0, A.ny_ + kb0, A.nz_ + kb0, RB(A.nt),
+        RB(A.sbx), RB(A.sby), RB(A.sbz), RB(A.sbt), RB(A.sax), RB(A.say), RB(A.saz), RB(A.sat),
+        i, j, k, kb0, smt, ib);
+    #undef RB
+  }
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -356,22 +419,21 @@ struct CorrectArgs {
   int rkstep;
   int mode;                     /* bit 0: velocity correction, bit 1: pressure update */
   unsigned long long * dtmin;   /* non-null: CFL minimum of the corrected field */
+  TileMap tm;
 };
 
-__device__ __forceinline__ void correct_body(const CorrectArgs & A, double * __restrict__ ux,
+__device__ __forceinline__ double correct_body(const CorrectArgs & A, double * __restrict__ ux,
     double * __restrict__ uy, double * __restrict__ uz, double * __restrict__ pp,
-    const double * __restrict__ psi){
+    const double * __restrict__ psi, const int i, const int j, const int k, const long long kb0){
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
   const bool inside = (i <= g.nx && j <= g.ny);
   double lmin = 1.;
   if(inside){
-    const long long o = tck_idx(g, i, j, k);
-    const long long sy = g.px;
-    const long long sz = g.plane;
+    /* pointers start at plane k-1 (kb0); 32-bit offsets inside the 3-plane window */
+    const int sy = g.px;
+    const int sz = (int)g.plane;
+    const int o = (i + TC_XOFF) + sy * j + sz;
     const double dt = *A.dt;
     const int s = A.rkstep;
     const double gamma = c_rka[s] + c_rkb[s];
@@ -382,18 +444,18 @@ __device__ __forceinline__ void correct_body(const CorrectArgs & A, double * __r
     if(i >= 2){ /* correct/ux.c:25-52 */
       const double hx = c.hxxf[i];
       const double v = ux[o] - prefactor / hx * (- psi[o - 1] + ps0);
-      tck_store_halo(ux, g, i, j, k, v, false);
+      tck_store_halo(ux - kb0, g, i, j, k, v, false);
       lmin = fmin(lmin, hx / (fabs(v) + small));
     }
     { /* correct/uy.c:25-52 */
       const double hy = c.hyxc[i];
       const double v = uy[o] - prefactor / hy * (- psi[o - sy] + ps0);
-      tck_store_halo(uy, g, i, j, k, v, false);
+      tck_store_halo(uy - kb0, g, i, j, k, v, false);
       lmin = fmin(lmin, hy / (fabs(v) + small));
     }
     { /* correct/uz.c:15-44 */
       const double v = uz[o] - prefactor / c.hz * (- psi[o - sz] + ps0);
-      tck_store_halo(uz, g, i, j, k, v, false);
+      tck_store_halo(uz - kb0, g, i, j, k, v, false);
       lmin = fmin(lmin, c.hz / (fabs(v) + small));
     }
     }
@@ -418,26 +480,43 @@ __device__ __forceinline__ void correct_body(const CorrectArgs & A, double * __r
         const double cc = - l - u;
         v -= 2. * pf / c.jdxc[i] * (+ l * psi[o - sz] + cc * ps0 + u * psi[o + sz]);
       }
-      tck_store_halo(pp, g, i, j, k, v, true);
+      tck_store_halo(pp - kb0, g, i, j, k, v, true);
     }
   }
-  if(A.dtmin){
-    /* CFL bound of decide_dt_adv (decide_dt.c:32-80) on the corrected velocity */
-    __shared__ double red[TCK_BLOCK / TCK_WAVE];
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    double m = tck_wave_min(lmin);
-    if((tid % TCK_WAVE) == 0) red[tid / TCK_WAVE] = m;
-    __syncthreads();
-    if(tid == 0){
-      const int nw = (blockDim.x * blockDim.y + TCK_WAVE - 1) / TCK_WAVE;
-      for(int w = 1; w < nw; w++) m = fmin(m, red[w]);
-      atomicMin(A.dtmin, tck_bits(m));
-    }
+  return lmin;
+}
+
+/* block minimum of v folded into one of the TCK_DTSLOTS accumulators */
+__device__ __forceinline__ void dt_block_min(double v, unsigned long long * dtmin){
+  __shared__ double red[TCK_BLOCK / TCK_WAVE];
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  double m = tck_wave_min(v);
+  if((tid % TCK_WAVE) == 0) red[tid / TCK_WAVE] = m;
+  __syncthreads();
+  if(tid == 0){
+    const int nw = (blockDim.x * blockDim.y + TCK_WAVE - 1) / TCK_WAVE;
+    for(int w = 1; w < nw; w++) m = fmin(m, red[w]);
+    const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    atomicMin(dtmin + (bid % TCK_DTSLOTS) * TCK_DTSTRIDE, tck_bits(m));
   }
 }
 
 __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
-  correct_body(A, A.ux, A.uy, A.uz, A.p, A.psi);
+  int bx, by, bz;
+  tile_of(A.tm, &bx, &by, &bz);
+  const int i = 1 + bx * TCK_WAVE + threadIdx.x;
+  const int j = 1 + by * blockDim.y + threadIdx.y;
+  const int k0 = 1 + bz * A.tm.kb;
+  const int k1 = min(k0 + A.tm.kb, A.g.nz + 1);
+  double lmin = 1.;
+  for(int k = k0; k < k1; k++){
+    int il = i;
+    asm volatile("" : "+v"(il));
+    const long long kb0 = (long long)(k - 1) * A.g.plane;
+    lmin = fmin(lmin, correct_body(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, A.p + kb0, A.psi + kb0, il, j, k, kb0));
+  }
+  /* CFL bound of decide_dt_adv (decide_dt.c:32-80) on the corrected velocity */
+  if(A.dtmin) dt_block_min(lmin, A.dtmin);
 }
 
 /* decide_dt_adv on the current field (decide_dt.c:32-80), for the first step and after
@@ -462,49 +541,62 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_dt_reduce(const DtArgs A){
     lmin = fmin(lmin, A.c.hyxc[i] / (fabs(A.uy[o]) + small));
     lmin = fmin(lmin, A.c.hz / (fabs(A.uz[o]) + small));
   }
-  __shared__ double red[TCK_BLOCK / TCK_WAVE];
-  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  double m = tck_wave_min(lmin);
-  if((tid % TCK_WAVE) == 0) red[tid / TCK_WAVE] = m;
-  __syncthreads();
-  if(tid == 0){
-    const int nw = (blockDim.x * blockDim.y + TCK_WAVE - 1) / TCK_WAVE;
-    for(int w = 1; w < nw; w++) m = fmin(m, red[w]);
-    atomicMin(A.dtmin, tck_bits(m));
-  }
+  dt_block_min(lmin, A.dtmin);
 }
 
 /* decide_dt.c:125-160: dt = coef_adv * min(1, CFL) then min with the explicit diffusive
- * limits (precomputed on the host, decide_dt.c:89-115); re-arms the CFL accumulator */
+ * limits (precomputed on the host, decide_dt.c:89-115); re-arms the CFL accumulators.
+ * One wave: lane t owns accumulator t. */
 __global__ void tck_dt_finalise(unsigned long long * dtmin, double * dt, double coef_adv,
     double dif_limit, double dt_fixed){
-  if(threadIdx.x != 0 || blockIdx.x != 0) return;
-  if(dt_fixed > 0.){
-    *dt = dt_fixed;
-  }else{
-    double v = __longlong_as_double((long long)*dtmin);
-    v *= coef_adv;
-    *dt = fmin(v, dif_limit);
+  const int t = threadIdx.x;
+  double v = 1.;
+  if(t < TCK_DTSLOTS){
+    v = __longlong_as_double((long long)dtmin[t * TCK_DTSTRIDE]);
+    dtmin[t * TCK_DTSTRIDE] = tck_bits(1.);
   }
-  *dtmin = tck_bits(1.);
+  v = tck_wave_min(v);
+  if(t == 0){
+    if(dt_fixed > 0.){
+      *dt = dt_fixed;
+    }else{
+      v *= coef_adv;
+      *dt = fmin(v, dif_limit);
+    }
+  }
 }
 
 __global__ void tck_dt_arm(unsigned long long * dtmin){
-  if(threadIdx.x == 0 && blockIdx.x == 0) *dtmin = tck_bits(1.);
+  if(threadIdx.x < TCK_DTSLOTS) dtmin[threadIdx.x * TCK_DTSTRIDE] = tck_bits(1.);
 }
 
 /* ------------------------------------------------------------------------------------- */
 extern "C" {
+
+static TileMap cell_tiles(const tc_geom_t & g, int kb, int ty){
+  TileMap T;
+  T.nbx = (g.nx + TCK_WAVE - 1) / TCK_WAVE;
+  T.nby = (g.ny + ty - 1) / ty;
+  T.kb = kb < 1 ? 1 : kb;
+  T.nbz = (g.nz + T.kb - 1) / T.kb;
+  T.total = (long long)T.nbx * T.nby * T.nbz;
+  return T;
+}
 
 static dim3 cell_grid(const tc_geom_t & g, dim3 * blk){
   *blk = dim3(64, 4, 1);
   return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
 }
 
+static int check_shape(int kb, int ty){
+  return (kb >= 1 && (ty == 1 || ty == 2 || ty == 4)) ? 0 : (int)hipErrorInvalidValue;
+}
+
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     const double * const cur[4], const double * p, double * const nxt[4],
     const double * const srcb[4], double * const srca[4], const double * dt, int rkstep,
-    int final_here, hipStream_t stream){
+    int final_here, int kb, int ty, hipStream_t stream){
+  if(check_shape(kb, ty)) return (int)hipErrorInvalidValue;
   PredictArgs A;
   A.g = *g; A.c = *c; A.f = *f;
   A.ux = cur[0]; A.uy = cur[1]; A.uz = cur[2]; A.t = cur[3]; A.p = p;
@@ -512,24 +604,23 @@ int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
   A.sbx = srcb[0]; A.sby = srcb[1]; A.sbz = srcb[2]; A.sbt = srcb[3];
   A.sax = srca[0]; A.say = srca[1]; A.saz = srca[2]; A.sat = srca[3];
   A.dt = dt; A.rkstep = rkstep; A.final_here = final_here;
-  dim3 blk;
-  dim3 grd = cell_grid(*g, &blk);
-  hipLaunchKernelGGL(tck_predict, grd, blk, 0, stream, A);
+  A.tm = cell_tiles(*g, kb, ty);
+  hipLaunchKernelGGL(tck_predict, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
 
 int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     double * ux, double * uy, double * uz, double * p, const double * psi, const double * dt,
-    int rkstep, int mode, unsigned long long * dtmin, hipStream_t stream){
+    int rkstep, int mode, unsigned long long * dtmin, int kb, int ty, hipStream_t stream){
+  if(check_shape(kb, ty)) return (int)hipErrorInvalidValue;
   CorrectArgs A;
   A.mode = mode;
   A.g = *g; A.c = *c; A.f = *f;
   A.ux = ux; A.uy = uy; A.uz = uz; A.p = p; A.psi = psi;
   A.dt = dt; A.rkstep = rkstep; A.dtmin = dtmin;
-  dim3 blk;
-  dim3 grd = cell_grid(*g, &blk);
-  hipLaunchKernelGGL(tck_correct, grd, blk, 0, stream, A);
+  A.tm = cell_tiles(*g, kb, ty);
+  hipLaunchKernelGGL(tck_correct, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -538,7 +629,7 @@ int tck_launch_dt(const tc_geom_t * g, const tc_coef_t * c, const double * ux, c
     const double * uz, unsigned long long * dtmin, hipStream_t stream){
   DtArgs A;
   A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.uz = uz; A.dtmin = dtmin;
-  hipLaunchKernelGGL(tck_dt_arm, dim3(1), dim3(1), 0, stream, dtmin);
+  hipLaunchKernelGGL(tck_dt_arm, dim3(1), dim3(TCK_WAVE), 0, stream, dtmin);
   TCK_CHECK_LAUNCH();
   dim3 blk;
   dim3 grd = cell_grid(*g, &blk);
@@ -549,7 +640,7 @@ int tck_launch_dt(const tc_geom_t * g, const tc_coef_t * c, const double * ux, c
 
 int tck_launch_dt_finalise(unsigned long long * dtmin, double * dt, double coef_adv,
     double dif_limit, double dt_fixed, hipStream_t stream){
-  hipLaunchKernelGGL(tck_dt_finalise, dim3(1), dim3(1), 0, stream, dtmin, dt, coef_adv, dif_limit, dt_fixed);
+  hipLaunchKernelGGL(tck_dt_finalise, dim3(1), dim3(TCK_WAVE), 0, stream, dtmin, dt, coef_adv, dif_limit, dt_fixed);
   TCK_CHECK_LAUNCH();
   return 0;
 }

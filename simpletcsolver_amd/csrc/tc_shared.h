@@ -25,6 +25,20 @@ typedef struct {
   long long n3;         /* plane * pz */
 } tc_geom_t;
 
+/* per-i coefficient tables, packed in one array of T_NTAB rows of tp = nx + 2 entries
+ * (row t at tab + t * tp); the rows used by the predictor come first (T_NPRED of them) so a
+ * kernel can stage them in LDS as one block */
+enum {
+  T_HXXF, T_HXXC, T_HYXF, T_HYXC, T_JDXF, T_JDXC,
+  T_AXF, T_IJDXF, T_IHXF, T_IHYF, T_JHXF, T_JHYF, T_JHZF, T_L1F, T_NUJF, T_N2JF, T_D1M, T_D1P,
+  T_AYC, T_AZC, T_IJDXC, T_IHYC, T_JHYC, T_JHZC, T_L1C, T_HDC, T_NUJC, T_N2JC, T_DAM, T_DAP,
+  T_LXU_L, T_LXU_C, T_LXU_U, T_LYU_L, T_LYU_C, T_LYU_U,
+  T_LXC_L, T_LXC_C, T_LXC_U, T_LYC_L, T_LYC_C, T_LYC_U,
+  T_PLX_L, T_PLX_U, T_PLY, T_PLZ,
+  T_NTAB
+};
+#define T_NPRED T_PLX_L
+
 /* per-i coefficient tables (device pointers).  Every table has nx+2 entries and is indexed
  * by the reference's i: face quantities at i = 1..nx+1, centre quantities at i = 0..nx+1.
  * Each entry is evaluated on the host with the operand order the reference uses inline, so
@@ -68,6 +82,8 @@ typedef struct {
   double lapz_l, lapz_c, lapz_u;
   double hz, ihz;
   double nu, kappa;
+  const double * tab;    /* all rows, T_NTAB x tp */
+  int tp;
 } tc_coef_t;
 
 /* boundary values and switches (src/param/*.c, overridable at run time) */
