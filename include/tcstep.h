@@ -25,6 +25,10 @@ extern "C" {
 #endif
 
 #define TC_FP64 0
+/* no reference counterpart: FP64 state (u, p), divergence, radial Poisson solve, projection
+ * and diagnostics; FP32 right-hand sides, RK sources, implicit-solve increments, FFTs and
+ * potential */
+#define TC_MIXED 1
 
 typedef struct {
   int nx, ny, nz;           /* glsizes (domain.c:40) */
@@ -38,7 +42,7 @@ typedef struct {
   double coef_dt_adv;       /* decide_dt.c:128 */
   double coef_dt_dif;       /* decide_dt.c:129 */
   double dt_fixed;          /* > 0: decide_dt returns this value (no reference counterpart) */
-  int precision;            /* TC_FP64 */
+  int precision;            /* TC_FP64 or TC_MIXED */
   int device;               /* HIP device ordinal */
 } tc_params_t;
 

@@ -28,6 +28,9 @@ EXPORTED = (
     "tc_tune_set",
 )
 
+# tc_params_t.precision (include/tcstep.h)
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
 # running statistics of statistics.c, in its accumulation order
 STATS = ("ux1", "ux2", "uy1", "uy2", "uz1", "uz2", "adv", "dif")
 
@@ -140,7 +143,7 @@ class Solver:
 
     def __init__(self, nx, ny, nz, xf, xc, is_curved=True, lengths=(1.0, 1.0, 1.0), Re=1.0,
                  Pr=1.0, implicit=(1, 0, 0), scalar=1, uy_walls=(1.0, 0.0), t_walls=(0.5, -0.5),
-                 coef_dt_adv=1.0, coef_dt_dif=1.0, dt_fixed=0.0, device=0):
+                 coef_dt_adv=1.0, coef_dt_dif=1.0, dt_fixed=0.0, device=0, precision="fp64"):
         lib = load_library()
         p = TcParams()
         lib.tc_params_default(ctypes.byref(p))
@@ -155,6 +158,10 @@ class Solver:
         p.coef_dt_adv, p.coef_dt_dif = float(coef_dt_adv), float(coef_dt_dif)
         p.dt_fixed = float(dt_fixed)
         p.device = int(device)
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision {precision!r}: one of {tuple(PRECISIONS)}")
+        p.precision = PRECISIONS[precision]
+        self.precision = precision
         self.params = p
         self.nx, self.ny, self.nz = nx, ny, nz
         xf = np.ascontiguousarray(xf, dtype=np.float64)

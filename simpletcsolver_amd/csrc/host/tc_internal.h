@@ -47,14 +47,19 @@ struct tc_solver_s {
   /* device state */
   double * u[TC_NFIELD][2];
   int cur[TC_NFIELD];
-  double * src[TC_NFIELD][2];
+  int mixed;              /* precision TC_MIXED: FP32 temporaries (src, increment, spectrum,
+                           * psi), FP64 state and projection arithmetic */
+  size_t esz;             /* bytes per temporary element: 8 or 4 */
+  void * src[TC_NFIELD][2];
+  void * dlt[TC_NFIELD];  /* the increment of the mixed step (FP64: the next field buffer) */
+  float * d_psi32;        /* psi of the mixed step */
   double * d_p;
   double * d_psi;
   double * d_tab;
-  double * d_spec;
+  void * d_spec;
   double * d_pr; double * d_pv; double * d_pl; double * d_pu;
   double * d_evy; double * d_evz;
-  double * d_twy; double * d_twyn; double * d_twz;
+  void * d_twy; void * d_twyn; void * d_twz;   /* double2, or float2 in the mixed step */
   double * d_dt;
   unsigned long long * d_dtmin;
   double * d_part;

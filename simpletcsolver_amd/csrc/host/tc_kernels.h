@@ -13,13 +13,15 @@
  * every TC_DTSTRIDE entries */
 #define TC_DTSLOTS 64
 #define TC_DTSTRIDE 16
+/* mixed: the temporaries (increment dlt, srcb, srca, psi) are float and the right-hand sides
+ * are evaluated in FP32; otherwise they are double and dlt may be nxt itself */
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
-    const double * const cur[4], const double * p, double * const nxt[4],
-    const double * const srcb[4], double * const srca[4], const double * dt, int rkstep,
-    int final_here, int kb, int ty, hipStream_t stream);
+    const double * const cur[4], const double * p, double * const nxt[4], void * const dlt[4],
+    const void * const srcb[4], void * const srca[4], const double * dt, int rkstep,
+    int final_here, int mixed, int kb, int ty, hipStream_t stream);
 int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
-    double * ux, double * uy, double * uz, double * p, const double * psi, const double * dt,
-    int rkstep, int mode, unsigned long long * dtmin, int kb, int ty, hipStream_t stream);
+    double * ux, double * uy, double * uz, double * p, const void * psi, const double * dt,
+    int rkstep, int mode, unsigned long long * dtmin, int mixed, int kb, int ty, hipStream_t stream);
 int tck_launch_dt(const tc_geom_t * g, const tc_coef_t * c, const double * ux, const double * uy,
     const double * uz, unsigned long long * dtmin, hipStream_t stream);
 int tck_launch_dt_finalise(unsigned long long * dtmin, double * dt, double coef_adv,
@@ -31,25 +33,28 @@ int tck_launch_factor(const double * dt, const int dirs[3], int nx, int ny, int 
     const double (*fac)[3], const int * i0, const int * n,
     const double * const (*lx)[3], const double * const (*ly)[3],
     double * const (*xt)[3], double * const (*yt)[4], double * const (*zt)[4], hipStream_t stream);
-int tck_launch_xsolve(const tc_geom_t * g, double * d, const double * ucur, const double * r,
-    const double * v, const double * l, int i0, int n, hipStream_t stream);
-int tck_launch_psolve(const tc_geom_t * g, int dir, double * d, const double * ucur,
+/* d: the increment (float when mixed); uout: where the final pass (ucur non-null) writes
+ * ucur + solution (FP64; may be d itself when not mixed) */
+int tck_launch_xsolve(const tc_geom_t * g, void * d, double * uout, const double * ucur,
+    const double * r, const double * v, const double * l, int i0, int n, int mixed, hipStream_t stream);
+int tck_launch_psolve(const tc_geom_t * g, int dir, void * d, double * uout, const double * ucur,
     const double * r, const double * v, const double * q1, const double * meta,
-    long long tstride, int tcol, int i0, int isz, hipStream_t stream);
+    long long tstride, int tcol, int i0, int isz, int mixed, hipStream_t stream);
 int tck_launch_pfactor(int nx, int nky, int nz, const double * pl, const double * pu,
     const double * hyxc, const double * evy, const double * evz, double hz, double * pr,
-    double * pv, hipStream_t stream);
-int tck_launch_pxsolve(int nx, int nlines, double * s, const double * pr, const double * pv,
-    const double * pl, hipStream_t stream);
+    double * pv, int exact_gauge, hipStream_t stream);
+int tck_launch_pxsolve(int nx, int nlines, void * s, const double * pr, const double * pv,
+    const double * pl, int mixed, hipStream_t stream);
 
 /* tck_fft.hip */
 int tck_fft_supported(int n);
+/* mixed: spectrum, transforms, twiddles and psi in FP32 */
 int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
-    const double * uy, const double * uz, const double * dt, int rkstep, double * S,
-    const double * twM, const double * twN, hipStream_t stream);
-int tck_launch_zfft(int nx, int nky, int nz, double * S, const double * tw, int sign, hipStream_t stream);
-int tck_launch_yifft_psi(const tc_geom_t * g, const double * S, double * psi, const double * twM,
-    const double * twN, hipStream_t stream);
+    const double * uy, const double * uz, const double * dt, int rkstep, void * S,
+    const void * twM, const void * twN, int mixed, hipStream_t stream);
+int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign, int mixed, hipStream_t stream);
+int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const void * twM,
+    const void * twN, int mixed, hipStream_t stream);
 
 /* tck_diag.hip */
 int tck_launch_copy(const double * src, double * dst, long long n, hipStream_t stream);

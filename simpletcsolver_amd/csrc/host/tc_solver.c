@@ -81,6 +81,13 @@ static int dalloc(tc_solver_t * s, double ** p, size_t n){
   return 0;
 }
 
+static int dalloc_bytes(tc_solver_t * s, void ** p, size_t bytes){
+  double * d = NULL;
+  if(dalloc(s, &d, (bytes + sizeof(double) - 1) / sizeof(double))) return 1;
+  *p = d;
+  return 0;
+}
+
 /* ---------------------------------------------------------------------------------------
  * metrics and per-i coefficient tables (domain.c:259-357, predict/*.c init_laplacians,
  * compute_potential.c:173-211), evaluated with the operand order of the reference
@@ -257,20 +264,26 @@ static int build_tables(tc_solver_t * s, const double * xf_in, const double * xc
   return 0;
 }
 
-/* FFT twiddles exp(-2 pi i t / n), t = 0..cnt-1 */
-static int upload_twiddles(tc_solver_t * s, int n, int cnt, double ** dptr){
+/* FFT twiddles exp(-2 pi i t / n), t = 0..cnt-1, evaluated in FP64 (rounded to FP32 for the
+ * mixed step) */
+static int upload_twiddles(tc_solver_t * s, int n, int cnt, void ** dptr){
   double * h = malloc(sizeof(double) * 2 * (size_t)cnt);
-  if(!h) return tc_fail("host allocation failed");
+  float * hf = malloc(sizeof(float) * 2 * (size_t)cnt);
+  if(!h || !hf){ free(h); free(hf); return tc_fail("host allocation failed"); }
   const double two_pi = 6.283185307179586476925286766559;
   for(int t = 0; t < cnt; t++){
     const double ang = - two_pi * (double)t / (double)n;
     h[2 * t] = cos(ang);
     h[2 * t + 1] = sin(ang);
+    hf[2 * t] = (float)h[2 * t];
+    hf[2 * t + 1] = (float)h[2 * t + 1];
   }
-  if(dalloc(s, dptr, 2 * (size_t)cnt)){ free(h); return 1; }
-  hipError_t e = hipMemcpyAsync(*dptr, h, sizeof(double) * 2 * cnt, hipMemcpyHostToDevice, s->stream);
+  const size_t bytes = 2 * s->esz * (size_t)cnt;
+  if(dalloc_bytes(s, dptr, bytes)){ free(h); free(hf); return 1; }
+  hipError_t e = hipMemcpyAsync(*dptr, s->mixed ? (void *)hf : (void *)h, bytes, hipMemcpyHostToDevice, s->stream);
   if(e == hipSuccess) e = hipStreamSynchronize(s->stream);
   free(h);
+  free(hf);
   if(e != hipSuccess) return tc_fail("twiddle upload: %s", hipGetErrorString(e));
   return 0;
 }
@@ -295,10 +308,10 @@ static int setup_poisson(tc_solver_t * s){
   free(evy);
   free(evz);
   const size_t nspec = (size_t)nx * nky * nz;
-  if(dalloc(s, &s->d_spec, 2 * nspec)) return 1;
+  if(dalloc_bytes(s, &s->d_spec, 2 * s->esz * nspec)) return 1;
   if(dalloc(s, &s->d_pr, nspec) || dalloc(s, &s->d_pv, nspec)) return 1;
   KOK(tck_launch_pfactor(nx, nky, nz, s->d_pl, s->d_pu, s->coef.hyxc, s->d_evy, s->d_evz,
-      s->hz, s->d_pr, s->d_pv, s->stream));
+      s->hz, s->d_pr, s->d_pv, s->mixed, s->stream));
   if(upload_twiddles(s, ny / 2, ny / 2, &s->d_twy)) return 1;
   if(upload_twiddles(s, ny, ny / 2 + 1, &s->d_twyn)) return 1;
   if(upload_twiddles(s, nz, nz, &s->d_twz)) return 1;
@@ -337,12 +350,14 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
     return tc_fail("ny/2 = %d and nz = %d must be products of 2 and 3", p->ny / 2, p->nz);
   if(p->nx > 1024) return tc_fail("nx = %d exceeds 1024 (radial line solver)", p->nx);
   if(p->ny / 2 > 4096 || p->nz > 4096) return tc_fail("transform length exceeds 4096");
-  if(p->precision != TC_FP64) return tc_fail("precision %d not supported", p->precision);
+  if(p->precision != TC_FP64 && p->precision != TC_MIXED) return tc_fail("precision %d not supported", p->precision);
   tc_solver_t * s = calloc(1, sizeof(tc_solver_t));
   if(!s) return tc_fail("host allocation failed");
   s->p = *p;
   s->tune[TC_TUNE_PREDICT][0] = 16; s->tune[TC_TUNE_PREDICT][1] = 4;
   s->tune[TC_TUNE_CORRECT][0] = 8; s->tune[TC_TUNE_CORRECT][1] = 4;
+  s->mixed = p->precision == TC_MIXED;
+  s->esz = s->mixed ? sizeof(float) : sizeof(double);
   if(hipSetDevice(p->device) != hipSuccess){
     free(s);
     return tc_fail("hipSetDevice(%d) failed", p->device);
@@ -369,11 +384,13 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
   for(int f = 0; f < TC_NFIELD && !rc; f++){
     rc |= dalloc(s, &s->u[f][0], n3);
     rc |= dalloc(s, &s->u[f][1], n3);
-    rc |= dalloc(s, &s->src[f][0], n3);
-    rc |= dalloc(s, &s->src[f][1], n3);
+    rc |= dalloc_bytes(s, &s->src[f][0], s->esz * n3);
+    rc |= dalloc_bytes(s, &s->src[f][1], s->esz * n3);
+    if(s->mixed) rc |= dalloc_bytes(s, &s->dlt[f], s->esz * n3);
   }
   if(!rc) rc |= dalloc(s, &s->d_p, n3);
-  if(!rc) rc |= dalloc(s, &s->d_psi, n3);
+  if(!rc && !s->mixed) rc |= dalloc(s, &s->d_psi, n3);
+  if(!rc && s->mixed) rc |= dalloc_bytes(s, (void **)&s->d_psi32, sizeof(float) * n3);
   if(!rc) rc |= dalloc(s, &s->d_dt, 4);
   if(!rc) rc |= dalloc(s, (double **)&s->d_dtmin, (size_t)TC_DTSLOTS * TC_DTSTRIDE);
   if(!rc){
@@ -389,7 +406,7 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
       for(int b = 0; b < 2 && !rc; b++) rc = tc_apply_bc(s, f, s->u[f][b]);
     }
     if(!rc) rc = tc_apply_bc(s, TC_FP, s->d_p);
-    if(!rc) rc = tc_apply_bc(s, TC_FPSI, s->d_psi);
+    if(!rc && !s->mixed) rc = tc_apply_bc(s, TC_FPSI, s->d_psi);
   }
   if(!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = tc_fail("initialisation failed");
   if(rc){
@@ -467,6 +484,7 @@ static void npy_x(const tc_solver_t * s, int f, int * nxs, int * i0){
 static int set_field_layout(tc_solver_t * s, const char * name, const double * data, int layout){
   const int f = field_id(name);
   if(f < 0) return tc_fail("unknown field '%s'", name);
+  if(f == TC_FPSI && s->mixed) return tc_fail("psi is an output of the mixed step and cannot be set");
   HIPOK(hipSetDevice(s->p.device));
   int nxs, i0;
   npy_x(s, f, &nxs, &i0);
@@ -505,10 +523,38 @@ int tc_finish_field(tc_solver_t * s, int f){
   return 0;
 }
 
+/* psi of the mixed step: FP32 rows, widened on the host */
+static int get_psi32(tc_solver_t * s, double * data, int layout){
+  const tc_geom_t * g = &s->g;
+  int nxs, i0;
+  npy_x(s, TC_FPSI, &nxs, &i0);
+  const size_t rows = layout == 0 ? (size_t)g->ny * g->nz : (size_t)g->py * g->pz;
+  float * tmp = malloc(sizeof(float) * rows * nxs);
+  if(!tmp) return tc_fail("host allocation failed");
+  const float * src = s->d_psi32 + i0 + TC_XOFF;
+  hipError_t e = hipSuccess;
+  if(layout == 0){
+    for(int k = 1; k <= g->nz && e == hipSuccess; k++){
+      e = hipMemcpy2DAsync(tmp + (size_t)(k - 1) * g->ny * nxs, sizeof(float) * nxs,
+          src + (size_t)g->px * (1 + (size_t)g->py * k), sizeof(float) * g->px,
+          sizeof(float) * nxs, g->ny, hipMemcpyDeviceToHost, s->stream);
+    }
+  }else{
+    e = hipMemcpy2DAsync(tmp, sizeof(float) * nxs, src, sizeof(float) * g->px, sizeof(float) * nxs,
+        rows, hipMemcpyDeviceToHost, s->stream);
+  }
+  if(e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if(e == hipSuccess) for(size_t q = 0; q < rows * nxs; q++) data[q] = (double)tmp[q];
+  free(tmp);
+  if(e != hipSuccess) return tc_fail("psi download: %s", hipGetErrorString(e));
+  return 0;
+}
+
 static int get_field_layout(tc_solver_t * s, const char * name, double * data, int layout){
   const int f = field_id(name);
   if(f < 0) return tc_fail("unknown field '%s'", name);
   HIPOK(hipSetDevice(s->p.device));
+  if(f == TC_FPSI && s->mixed) return get_psi32(s, data, layout);
   int nxs, i0;
   npy_x(s, f, &nxs, &i0);
   const tc_geom_t * g = &s->g;
@@ -707,16 +753,18 @@ static int predict(tc_solver_t * s, int rkstep){
   }
   const double * cur[4];
   double * nxt[4];
-  const double * srcb[4];
-  double * srca[4];
+  void * dlt[4];
+  const void * srcb[4];
+  void * srca[4];
   for(int f = 0; f < 4; f++){
     cur[f] = s->u[f][s->cur[f]];
     nxt[f] = s->u[f][1 - s->cur[f]];
+    dlt[f] = s->mixed ? s->dlt[f] : (void *)nxt[f];
     srcb[f] = rkstep > 0 ? s->src[f][(rkstep - 1) & 1] : NULL;
     srca[f] = rkstep < 2 ? s->src[f][rkstep & 1] : NULL;
   }
-  PROF(TC_S_PREDICT, tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, srcb, srca,
-      s->d_dt, rkstep, !any_implicit, s->tune[TC_TUNE_PREDICT][0], s->tune[TC_TUNE_PREDICT][1], s->stream));
+  PROF(TC_S_PREDICT, tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, dlt, srcb, srca,
+      s->d_dt, rkstep, !any_implicit, s->mixed, s->tune[TC_TUNE_PREDICT][0], s->tune[TC_TUNE_PREDICT][1], s->stream));
   if(any_implicit){
     int last = 0;
     for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
@@ -726,15 +774,16 @@ static int predict(tc_solver_t * s, int rkstep){
       const int i0 = ux ? 2 : 1;
       const int n = ux ? s->p.nx - 1 : s->p.nx;
       if(s->p.implicit[0]){
-        PROF(TC_S_XSOLVE, tck_launch_xsolve(&s->g, nxt[f], last == 0 ? cur[f] : NULL, L->xr, L->xv, L->xl, i0, n, s->stream));
+        PROF(TC_S_XSOLVE, tck_launch_xsolve(&s->g, dlt[f], nxt[f], last == 0 ? cur[f] : NULL, L->xr, L->xv,
+            L->xl, i0, n, s->mixed, s->stream));
       }
       if(s->p.implicit[1]){
-        PROF(TC_S_YSOLVE, tck_launch_psolve(&s->g, 1, nxt[f], last == 1 ? cur[f] : NULL, L->yr, L->yv, L->yq1,
-            L->ymeta, s->tp, 1, i0, n, s->stream));
+        PROF(TC_S_YSOLVE, tck_launch_psolve(&s->g, 1, dlt[f], nxt[f], last == 1 ? cur[f] : NULL, L->yr, L->yv,
+            L->yq1, L->ymeta, s->tp, 1, i0, n, s->mixed, s->stream));
       }
       if(s->p.implicit[2]){
-        PROF(TC_S_ZSOLVE, tck_launch_psolve(&s->g, 2, nxt[f], cur[f], L->zr, L->zv, L->zq1, L->zmeta, 1, 0,
-            i0, n, s->stream));
+        PROF(TC_S_ZSOLVE, tck_launch_psolve(&s->g, 2, dlt[f], nxt[f], cur[f], L->zr, L->zv, L->zq1, L->zmeta, 1, 0,
+            i0, n, s->mixed, s->stream));
       }
     }
   }
@@ -744,20 +793,23 @@ static int predict(tc_solver_t * s, int rkstep){
 
 static int potential(tc_solver_t * s, int rkstep){
   const tc_geom_t * g = &s->g;
+  const int mx = s->mixed;
+  void * psi = mx ? (void *)s->d_psi32 : (void *)s->d_psi;
   PROF(TC_S_DIVY, tck_launch_div_yfft(g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
-      tc_field_ptr(s, TC_FUZ), s->d_dt, rkstep, s->d_spec, s->d_twy, s->d_twyn, s->stream));
-  PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, -1, s->stream));
-  PROF(TC_S_PXSOLVE, tck_launch_pxsolve(g->nx, s->nky * g->nz, s->d_spec, s->d_pr, s->d_pv, s->d_pl, s->stream));
-  PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, +1, s->stream));
-  PROF(TC_S_YIFFT, tck_launch_yifft_psi(g, s->d_spec, s->d_psi, s->d_twy, s->d_twyn, s->stream));
+      tc_field_ptr(s, TC_FUZ), s->d_dt, rkstep, s->d_spec, s->d_twy, s->d_twyn, mx, s->stream));
+  PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, -1, mx, s->stream));
+  PROF(TC_S_PXSOLVE, tck_launch_pxsolve(g->nx, s->nky * g->nz, s->d_spec, s->d_pr, s->d_pv, s->d_pl, mx, s->stream));
+  PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, +1, mx, s->stream));
+  PROF(TC_S_YIFFT, tck_launch_yifft_psi(g, s->d_spec, psi, s->d_twy, s->d_twyn, mx, s->stream));
   return 0;
 }
 
 /* mode: 1 velocity correction (correct/*.c), 2 pressure update (update_pressure.c), 3 both */
 static int correct(tc_solver_t * s, int rkstep, int mode, int want_cfl){
   PROF(TC_S_CORRECT, tck_launch_correct(&s->g, &s->coef, &s->flags, tc_field_ptr(s, TC_FUX),
-      tc_field_ptr(s, TC_FUY), tc_field_ptr(s, TC_FUZ), s->d_p, s->d_psi, s->d_dt, rkstep, mode,
-      want_cfl ? s->d_dtmin : NULL, s->tune[TC_TUNE_CORRECT][0], s->tune[TC_TUNE_CORRECT][1], s->stream));
+      tc_field_ptr(s, TC_FUY), tc_field_ptr(s, TC_FUZ), s->d_p,
+      s->mixed ? (const void *)s->d_psi32 : (const void *)s->d_psi, s->d_dt, rkstep, mode,
+      want_cfl ? s->d_dtmin : NULL, s->mixed, s->tune[TC_TUNE_CORRECT][0], s->tune[TC_TUNE_CORRECT][1], s->stream));
   return 0;
 }
 

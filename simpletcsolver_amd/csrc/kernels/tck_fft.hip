@@ -27,77 +27,98 @@
 #define FFT_EPT 16   /* complex elements per thread per pass */
 #define FFT_MAXTHREADS 1024
 
+/* complex pair of the transform's arithmetic type T (double, or float in the mixed step) */
+template <typename T> struct cvec;
+template <> struct cvec<double> { typedef double2 type; };
+template <> struct cvec<float> { typedef float2 type; };
+template <typename T> __device__ __forceinline__ typename cvec<T>::type mk2(T x, T y);
+template <> __device__ __forceinline__ double2 mk2<double>(double x, double y){ return make_double2(x, y); }
+template <> __device__ __forceinline__ float2 mk2<float>(float x, float y){ return make_float2(x, y); }
+
 struct FftPlan {
   int M;
   int nst;
   int rad[FFT_MAXST];
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b){
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename V> __device__ __forceinline__ V cmul(V a, V b){
+  V r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b){ return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b){ return make_double2(a.x - b.x, a.y - b.y); }
+template <typename V> __device__ __forceinline__ V cadd(V a, V b){ V r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename V> __device__ __forceinline__ V csub(V a, V b){ V r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
 /* s i a */
-__device__ __forceinline__ double2 crot(double2 a, double s){ return make_double2(- s * a.y, s * a.x); }
-__device__ __forceinline__ double2 cconj(double2 a){ return make_double2(a.x, - a.y); }
+template <typename V, typename T> __device__ __forceinline__ V crot(V a, T s){ V r; r.x = - s * a.y; r.y = s * a.x; return r; }
+template <typename V> __device__ __forceinline__ V cconj(V a){ V r; r.x = a.x; r.y = - a.y; return r; }
 
 /* R-point DFTs, y_k = sum_n x_n exp(s 2 pi i n k / R) */
-template <int R> __device__ __forceinline__ void dft(double2 * x, double s);
+template <int R, typename T> struct dftr;
 
-template <> __device__ __forceinline__ void dft<2>(double2 * x, double s){
-  (void)s;
-  const double2 a = x[0], b = x[1];
-  x[0] = cadd(a, b);
-  x[1] = csub(a, b);
-}
-
-template <> __device__ __forceinline__ void dft<4>(double2 * x, double s){
-  const double2 a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
-  const double2 b0 = cadd(x[1], x[3]), b1 = crot(csub(x[1], x[3]), s);
-  x[0] = cadd(a0, b0);
-  x[2] = csub(a0, b0);
-  x[1] = cadd(a1, b1);
-  x[3] = csub(a1, b1);
-}
-
-template <> __device__ __forceinline__ void dft<3>(double2 * x, double s){
-  const double h = 0.86602540378443864676;   /* sqrt(3) / 2 */
-  const double2 t1 = cadd(x[1], x[2]);
-  const double2 t2 = crot(csub(x[1], x[2]), s * h);
-  const double2 m = make_double2(x[0].x - 0.5 * t1.x, x[0].y - 0.5 * t1.y);
-  x[0] = cadd(x[0], t1);
-  x[1] = cadd(m, t2);
-  x[2] = csub(m, t2);
-}
-
-template <> __device__ __forceinline__ void dft<8>(double2 * x, double s){
-  const double r = 0.70710678118654752440;   /* sqrt(1/2) */
-  double2 e[4] = {x[0], x[2], x[4], x[6]};
-  double2 o[4] = {x[1], x[3], x[5], x[7]};
-  dft<4>(e, s);
-  dft<4>(o, s);
-  /* o_k *= w8^k, w8 = r (1 + s i) */
-  o[1] = make_double2(r * (o[1].x - s * o[1].y), r * (o[1].y + s * o[1].x));
-  o[2] = crot(o[2], s);
-  o[3] = make_double2(r * (- o[3].x - s * o[3].y), r * (- o[3].y + s * o[3].x));
-  #pragma unroll
-  for(int k = 0; k < 4; k++){
-    x[k] = cadd(e[k], o[k]);
-    x[k + 4] = csub(e[k], o[k]);
+template <typename T> struct dftr<2, T> {
+  typedef typename cvec<T>::type V;
+  static __device__ __forceinline__ void run(V * x, T s){
+    (void)s;
+    const V a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
   }
-}
+};
 
-template <int R>
-__device__ __forceinline__ void fft_stage(double2 * buf, int M, int ld, int nb, int Ns,
-    const double2 * __restrict__ tw, double s){
+template <typename T> struct dftr<4, T> {
+  typedef typename cvec<T>::type V;
+  static __device__ __forceinline__ void run(V * x, T s){
+    const V a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+    const V b0 = cadd(x[1], x[3]), b1 = crot(csub(x[1], x[3]), s);
+    x[0] = cadd(a0, b0);
+    x[2] = csub(a0, b0);
+    x[1] = cadd(a1, b1);
+    x[3] = csub(a1, b1);
+  }
+};
+
+template <typename T> struct dftr<3, T> {
+  typedef typename cvec<T>::type V;
+  static __device__ __forceinline__ void run(V * x, T s){
+    const T h = T(0.86602540378443864676);   /* sqrt(3) / 2 */
+    const V t1 = cadd(x[1], x[2]);
+    const V t2 = crot(csub(x[1], x[2]), s * h);
+    const V m = mk2<T>(x[0].x - T(0.5) * t1.x, x[0].y - T(0.5) * t1.y);
+    x[0] = cadd(x[0], t1);
+    x[1] = cadd(m, t2);
+    x[2] = csub(m, t2);
+  }
+};
+
+template <typename T> struct dftr<8, T> {
+  typedef typename cvec<T>::type V;
+  static __device__ __forceinline__ void run(V * x, T s){
+    const T r = T(0.70710678118654752440);   /* sqrt(1/2) */
+    V e[4] = {x[0], x[2], x[4], x[6]};
+    V o[4] = {x[1], x[3], x[5], x[7]};
+    dftr<4, T>::run(e, s);
+    dftr<4, T>::run(o, s);
+    /* o_k *= w8^k, w8 = r (1 + s i) */
+    o[1] = mk2<T>(r * (o[1].x - s * o[1].y), r * (o[1].y + s * o[1].x));
+    o[2] = crot(o[2], s);
+    o[3] = mk2<T>(r * (- o[3].x - s * o[3].y), r * (- o[3].y + s * o[3].x));
+    #pragma unroll
+    for(int k = 0; k < 4; k++){
+      x[k] = cadd(e[k], o[k]);
+      x[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+template <int R, typename T>
+__device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, int ld, int nb, int Ns,
+    const typename cvec<T>::type * __restrict__ tw, T s){
+  typedef typename cvec<T>::type V;
   constexpr int BPT = (FFT_EPT + R - 1) / R;
   const int nt = blockDim.x;
   const int tid = threadIdx.x;
   const int per = M / R;
   const int nbf = nb * per;
   const int twstep = M / (Ns * R);
-  double2 v[BPT][R];
+  V v[BPT][R];
   #pragma unroll
   for(int q = 0; q < BPT; q++){
     const int bf = tid + q * nt;
@@ -105,18 +126,18 @@ __device__ __forceinline__ void fft_stage(double2 * buf, int M, int ld, int nb, 
       const int b = bf / per;
       const int jj = bf - b * per;
       const int jm = jj % Ns;
-      const double2 * src = buf + b * ld + jj;
+      const V * src = buf + b * ld + jj;
       #pragma unroll
       for(int r = 0; r < R; r++){
-        double2 x = src[r * per];
+        V x = src[r * per];
         if(r > 0){
-          double2 w = tw[jm * r * twstep];
-          if(s > 0.) w.y = - w.y;
+          V w = tw[jm * r * twstep];
+          if(s > T(0.)) w.y = - w.y;
           x = cmul(x, w);
         }
         v[q][r] = x;
       }
-      dft<R>(v[q], s);
+      dftr<R, T>::run(v[q], s);
     }
   }
   __syncthreads();
@@ -127,7 +148,7 @@ __device__ __forceinline__ void fft_stage(double2 * buf, int M, int ld, int nb, 
       const int b = bf / per;
       const int jj = bf - b * per;
       const int jm = jj % Ns;
-      double2 * dst = buf + b * ld + (jj - jm) * R + jm;
+      V * dst = buf + b * ld + (jj - jm) * R + jm;
       #pragma unroll
       for(int r = 0; r < R; r++) dst[r * Ns] = v[q][r];
     }
@@ -136,15 +157,16 @@ __device__ __forceinline__ void fft_stage(double2 * buf, int M, int ld, int nb, 
 }
 
 /* nb transforms of length P.M stored at buf + b * ld; tw[t] = exp(-2 pi i t / M) */
-__device__ void lds_fft(double2 * buf, int ld, int nb, const FftPlan & P,
-    const double2 * __restrict__ tw, double s){
+template <typename T>
+__device__ void lds_fft(typename cvec<T>::type * buf, int ld, int nb, const FftPlan & P,
+    const typename cvec<T>::type * __restrict__ tw, T s){
   int Ns = 1;
   for(int st = 0; st < P.nst; st++){
     const int R = P.rad[st];
-    if(R == 8) fft_stage<8>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(R == 4) fft_stage<4>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(R == 3) fft_stage<3>(buf, P.M, ld, nb, Ns, tw, s);
-    else fft_stage<2>(buf, P.M, ld, nb, Ns, tw, s);
+    if(R == 8) fft_stage<8, T>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(R == 4) fft_stage<4, T>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(R == 3) fft_stage<3, T>(buf, P.M, ld, nb, Ns, tw, s);
+    else fft_stage<2, T>(buf, P.M, ld, nb, Ns, tw, s);
     Ns *= R;
   }
 }
@@ -159,15 +181,21 @@ struct YFwdArgs {
   const double * ux; const double * uy; const double * uz;
   const double * dt;
   int rkstep;
-  double2 * S;
-  int nky, ti;
+  void * S;                      /* spectrum of the transform type */
+  int nky, ti, ld;               /* ld: LDS row pitch, M + 1 (odd, so the rows of a tile
+                                  * start in different banks) */
   FftPlan P;                     /* length M = ny / 2 */
-  const double2 * twM;           /* exp(-2 pi i t / M) */
-  const double2 * twN;           /* exp(-2 pi i k / ny), k = 0..M */
+  const void * twM;              /* exp(-2 pi i t / M) */
+  const void * twN;              /* exp(-2 pi i k / ny), k = 0..M */
 };
 
+/* the divergence is formed in FP64 (assign_input :404-423) and rounded to the transform
+ * type T only when it enters the transform */
+template <typename T>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A){
-  extern __shared__ double2 sm[];
+  typedef typename cvec<T>::type V;
+  extern __shared__ __align__(16) unsigned char fft_smem[];
+  V * sm = (V *)fft_smem;
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
   const int ti = A.ti;
@@ -180,7 +208,9 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
   const double prefactor = 1. / (gamma * dt) / norm;
   const long long sy = g.px;
   const long long sz = g.plane;
-  /* divergence (assign_input :404-423) packed as z_m = x_2m + i x_2m+1 */
+  const V * twM = (const V *)A.twM;
+  const V * twN = (const V *)A.twN;
+  /* divergence packed as z_m = x_2m + i x_2m+1 */
   for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
     const int t = e % ti;
     const int j0 = e / ti;
@@ -194,109 +224,122 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
           - c.jhzc[i] * A.uz[o] + c.jhzc[i] * A.uz[o + sz]);
       val = prefactor * div;
     }
-    ((double *)(sm + t * M))[j0] = val;
+    ((T *)(sm + t * A.ld))[j0] = T(val);
   }
   __syncthreads();
-  lds_fft(sm, M, ti, A.P, A.twM, -1.);
+  lds_fft<T>(sm, A.ld, ti, A.P, twM, T(-1.));
   /* split: X_k = (Z_k + conj Z_{M-k}) / 2 - i W^k (Z_k - conj Z_{M-k}) / 2, k = 0..M */
+  V * S = (V *)A.S;
   for(int e = threadIdx.x; e < ti * (M + 1); e += blockDim.x){
     const int t = e % ti;
     const int kk = e / ti;
     const int i = i0 + t;
     if(i > g.nx) continue;
-    const double2 zk = sm[t * M + (kk == M ? 0 : kk)];
-    const double2 zc = cconj(sm[t * M + (kk == 0 ? 0 : M - kk)]);
-    const double2 w = A.twN[kk];
-    const double2 ev = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
-    const double2 od = make_double2(0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y));
+    const V zk = sm[t * A.ld + (kk == M ? 0 : kk)];
+    const V zc = cconj(sm[t * A.ld + (kk == 0 ? 0 : M - kk)]);
+    const V w = twN[kk];
+    const V ev = mk2<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+    const V od = mk2<T>(T(0.5) * (zk.x - zc.x), T(0.5) * (zk.y - zc.y));
     /* - i w od */
-    const double2 wo = cmul(w, od);
-    const double2 X = make_double2(ev.x + wo.y, ev.y - wo.x);
-    A.S[(long long)(i - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))] = X;
+    const V wo = cmul(w, od);
+    const V X = mk2<T>(ev.x + wo.y, ev.y - wo.x);
+    S[(long long)(i - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))] = X;
   }
 }
 
 /* ------------------------------------------------------------------------------------- */
 struct ZArgs {
-  int nx, nky, nz, tz;
-  double2 * S;
+  int nx, nky, nz, tz, ld;
+  void * S;
   FftPlan P;               /* length nz */
-  const double2 * tw;
+  const void * tw;
   double sign;
 };
 
+template <typename T>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
-  extern __shared__ double2 sm[];
+  typedef typename cvec<T>::type V;
+  extern __shared__ __align__(16) unsigned char fft_smem[];
+  V * sm = (V *)fft_smem;
   const int tz = A.tz;
   const int nz = A.nz;
   const int i0 = blockIdx.x * tz;
   const int ky = blockIdx.y;
   const long long kstride = (long long)A.nx * A.nky;
-  double2 * base = A.S + (long long)A.nx * ky;
+  V * base = (V *)A.S + (long long)A.nx * ky;
   for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
     const int t = e % tz;
     const int kz = e / tz;
     const int ii = i0 + t;
-    sm[t * nz + kz] = ii < A.nx ? base[ii + kstride * kz] : make_double2(0., 0.);
+    sm[t * A.ld + kz] = ii < A.nx ? base[ii + kstride * kz] : mk2<T>(T(0.), T(0.));
   }
   __syncthreads();
-  lds_fft(sm, nz, tz, A.P, A.tw, A.sign);
+  lds_fft<T>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
   for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
     const int t = e % tz;
     const int kz = e / tz;
     const int ii = i0 + t;
-    if(ii < A.nx) base[ii + kstride * kz] = sm[t * nz + kz];
+    if(ii < A.nx) base[ii + kstride * kz] = sm[t * A.ld + kz];
   }
 }
 
 /* ------------------------------------------------------------------------------------- */
 struct YInvArgs {
   tc_geom_t g;
-  const double2 * S;
-  double * psi;
-  int nky, ti;
+  const void * S;
+  void * psi;              /* of the transform type */
+  int nky, ti, ld;
   FftPlan P;
-  const double2 * twM;
-  const double2 * twN;
+  const void * twM;
+  const void * twN;
 };
 
 /* Z'_k = (X_k + conj X_{M-k}) + i W^{-k} (X_k - conj X_{M-k}) so that
  * IFFT_M(Z')_m = x_2m + i x_2m+1 (unnormalised, as FFTW's c2r) */
-__device__ __forceinline__ double2 c2r_pre(double2 xk, double2 xmk, double2 w){
-  const double2 xc = cconj(xmk);
-  const double2 a = cadd(xk, xc);
-  const double2 d = csub(xk, xc);
-  const double2 wd = cmul(cconj(w), d);
-  return make_double2(a.x - wd.y, a.y + wd.x);
+template <typename V>
+__device__ __forceinline__ V c2r_pre(V xk, V xmk, V w){
+  const V xc = cconj(xmk);
+  const V a = cadd(xk, xc);
+  const V d = csub(xk, xc);
+  const V wd = cmul(cconj(w), d);
+  V r;
+  r.x = a.x - wd.y;
+  r.y = a.y + wd.x;
+  return r;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A){
-  extern __shared__ double2 sm[];
+  typedef typename cvec<T>::type V;
+  extern __shared__ __align__(16) unsigned char fft_smem[];
+  V * sm = (V *)fft_smem;
   const tc_geom_t & g = A.g;
   const int ti = A.ti;
   const int M = A.P.M;
   const int i0 = 1 + blockIdx.x * ti;
   const int k = 1 + blockIdx.y;
   const long long rowbase = (long long)g.nx * (long long)A.nky * (k - 1);
+  const V * S = (const V *)A.S;
+  const V * twN = (const V *)A.twN;
   /* load X_1..X_{M-1}; Z'_0 needs X_0 and X_M only and is formed here */
   for(int e = threadIdx.x; e < ti * M; e += blockDim.x){
     const int t = e % ti;
     const int kk = e / ti;
     const int i = i0 + t;
-    double2 v = make_double2(0., 0.);
+    V v = mk2<T>(T(0.), T(0.));
     if(i <= g.nx){
       const long long o = (long long)(i - 1) + rowbase;
       if(kk == 0){
-        double2 x0 = A.S[o];
-        double2 xm = A.S[o + (long long)g.nx * M];
-        x0.y = 0.;
-        xm.y = 0.;
-        v = c2r_pre(x0, xm, A.twN[0]);
+        V x0 = S[o];
+        V xm = S[o + (long long)g.nx * M];
+        x0.y = T(0.);
+        xm.y = T(0.);
+        v = c2r_pre(x0, xm, twN[0]);
       }else{
-        v = A.S[o + (long long)g.nx * kk];
+        v = S[o + (long long)g.nx * kk];
       }
     }
-    sm[t * M + kk] = v;
+    sm[t * A.ld + kk] = v;
   }
   __syncthreads();
   /* pairs (k, M-k), k = 1..M/2, in place */
@@ -305,24 +348,25 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     const int t = e % ti;
     const int k1 = 1 + e / ti;
     const int k2 = M - k1;
-    const double2 xa = sm[t * M + k1];
-    const double2 xb = sm[t * M + k2];
-    const double2 z1 = c2r_pre(xa, xb, A.twN[k1]);
+    const V xa = sm[t * A.ld + k1];
+    const V xb = sm[t * A.ld + k2];
+    const V z1 = c2r_pre(xa, xb, twN[k1]);
     if(k2 != k1){
-      const double2 z2 = c2r_pre(xb, xa, A.twN[k2]);
-      sm[t * M + k2] = z2;
+      const V z2 = c2r_pre(xb, xa, twN[k2]);
+      sm[t * A.ld + k2] = z2;
     }
-    sm[t * M + k1] = z1;
+    sm[t * A.ld + k1] = z1;
   }
   __syncthreads();
-  lds_fft(sm, M, ti, A.P, A.twM, 1.);
+  lds_fft<T>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
+  T * psi = (T *)A.psi;
   for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
     const int t = e % ti;
     const int jj = e / ti;
     const int i = i0 + t;
     if(i > g.nx) continue;
-    const double v = ((const double *)(sm + t * M))[jj];
-    tck_store_halo(A.psi, g, i, jj + 1, k, v, true);
+    const T v = ((const T *)(sm + t * A.ld))[jj];
+    tck_store_halo(psi, g, i, jj + 1, k, v, true);
   }
 }
 
@@ -370,9 +414,9 @@ static size_t lds_limit(void){
   return g_lds_limit;
 }
 
-static int fft_tile(int M, int cap, int * threads){
+static int fft_tile(int M, int cap, size_t csize, int * threads){
   int b = 16;
-  while(b > 1 && ((size_t)b * M * sizeof(double2) > lds_limit() || b * M > FFT_EPT * FFT_MAXTHREADS)) b /= 2;
+  while(b > 1 && ((size_t)b * (M + 1) * csize > lds_limit() || b * M > FFT_EPT * FFT_MAXTHREADS)) b /= 2;
   while(b * 2 <= cap && b * 2 * M <= FFT_EPT * 64) b *= 2;   /* short transforms: wider tiles */
   if(b > cap) b = cap < 1 ? 1 : cap;
   int t = (b * M + FFT_EPT - 1) / FFT_EPT;
@@ -384,7 +428,7 @@ static int fft_tile(int M, int cap, int * threads){
 
 int tck_fft_batch(int M, int cap){
   int t;
-  return fft_tile(M, cap, &t);
+  return fft_tile(M, cap, sizeof(double2), &t);
 }
 
 static int set_lds(const void * fn, size_t bytes){
@@ -392,48 +436,73 @@ static int set_lds(const void * fn, size_t bytes){
   return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess ? 0 : -1;
 }
 
+/* mixed: spectrum, transforms and psi in FP32 (twiddles twM, twN, tw as float2) */
 int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
-    const double * uy, const double * uz, const double * dt, int rkstep, double * S,
-    const double * twM, const double * twN, hipStream_t stream){
+    const double * uy, const double * uz, const double * dt, int rkstep, void * S,
+    const void * twM, const void * twN, int mixed, hipStream_t stream){
   YFwdArgs A;
   A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.uz = uz; A.dt = dt; A.rkstep = rkstep;
-  A.S = (double2 *)S; A.nky = g->ny / 2 + 1;
+  A.S = S; A.nky = g->ny / 2 + 1;
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
+  const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.ti = fft_tile(A.P.M, 64, &nt);
-  A.twM = (const double2 *)twM; A.twN = (const double2 *)twN;
-  const size_t lds = sizeof(double2) * (size_t)A.ti * A.P.M;
-  if(set_lds((const void *)tck_div_yfft, lds)) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(tck_div_yfft, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(nt), lds, stream, A);
+  A.ti = fft_tile(A.P.M, 64, cs, &nt);
+  A.twM = twM; A.twN = twN;
+  A.ld = A.P.M + 1;
+  const size_t lds = cs * (size_t)A.ti * A.ld;
+  const dim3 grd((g->nx + A.ti - 1) / A.ti, g->nz);
+  if(mixed){
+    if(set_lds((const void *)tck_div_yfft<float>, lds)) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(tck_div_yfft<float>, grd, dim3(nt), lds, stream, A);
+  }else{
+    if(set_lds((const void *)tck_div_yfft<double>, lds)) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(tck_div_yfft<double>, grd, dim3(nt), lds, stream, A);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }
 
-int tck_launch_zfft(int nx, int nky, int nz, double * S, const double * tw, int sign, hipStream_t stream){
+int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign, int mixed, hipStream_t stream){
   ZArgs A;
-  A.nx = nx; A.nky = nky; A.nz = nz; A.S = (double2 *)S; A.tw = (const double2 *)tw;
+  A.nx = nx; A.nky = nky; A.nz = nz; A.S = S; A.tw = tw;
   A.sign = sign < 0 ? -1. : 1.;
   if(make_plan(nz, &A.P) != 0) return (int)hipErrorInvalidValue;
+  const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.tz = fft_tile(nz, 64, &nt);
-  const size_t lds = sizeof(double2) * (size_t)A.tz * nz;
-  if(set_lds((const void *)tck_zfft, lds)) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(tck_zfft, dim3((nx + A.tz - 1) / A.tz, nky), dim3(nt), lds, stream, A);
+  A.tz = fft_tile(nz, 64, cs, &nt);
+  A.ld = nz + 1;
+  const size_t lds = cs * (size_t)A.tz * A.ld;
+  const dim3 grd((nx + A.tz - 1) / A.tz, nky);
+  if(mixed){
+    if(set_lds((const void *)tck_zfft<float>, lds)) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(tck_zfft<float>, grd, dim3(nt), lds, stream, A);
+  }else{
+    if(set_lds((const void *)tck_zfft<double>, lds)) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(tck_zfft<double>, grd, dim3(nt), lds, stream, A);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }
 
-int tck_launch_yifft_psi(const tc_geom_t * g, const double * S, double * psi, const double * twM,
-    const double * twN, hipStream_t stream){
+int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const void * twM,
+    const void * twN, int mixed, hipStream_t stream){
   YInvArgs A;
-  A.g = *g; A.S = (const double2 *)S; A.psi = psi; A.nky = g->ny / 2 + 1;
+  A.g = *g; A.S = S; A.psi = psi; A.nky = g->ny / 2 + 1;
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
+  const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.ti = fft_tile(A.P.M, 64, &nt);
-  A.twM = (const double2 *)twM; A.twN = (const double2 *)twN;
-  const size_t lds = sizeof(double2) * (size_t)A.ti * A.P.M;
-  if(set_lds((const void *)tck_yifft_psi, lds)) return (int)hipErrorInvalidValue;
-  hipLaunchKernelGGL(tck_yifft_psi, dim3((g->nx + A.ti - 1) / A.ti, g->nz), dim3(nt), lds, stream, A);
+  A.ti = fft_tile(A.P.M, 64, cs, &nt);
+  A.twM = twM; A.twN = twN;
+  A.ld = A.P.M + 1;
+  const size_t lds = cs * (size_t)A.ti * A.ld;
+  const dim3 grd((g->nx + A.ti - 1) / A.ti, g->nz);
+  if(mixed){
+    if(set_lds((const void *)tck_yifft_psi<float>, lds)) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(tck_yifft_psi<float>, grd, dim3(nt), lds, stream, A);
+  }else{
+    if(set_lds((const void *)tck_yifft_psi<double>, lds)) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(tck_yifft_psi<double>, grd, dim3(nt), lds, stream, A);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }

@@ -329,7 +329,8 @@ static int num_cus(void){
  * final: store ucur + solution with halos (the field update of ux.c:544-556). */
 struct XSolveArgs {
   tc_geom_t g;
-  double * d;
+  void * d;              /* the increment: double, or float in the mixed step */
+  double * uout;         /* final pass: the new field (the increment's buffer itself in FP64) */
   const double * ucur;
   const double * r; const double * v; const double * l;
   int i0, n;
@@ -340,10 +341,10 @@ struct XSolveArgs {
  * per block into registers.  Lines are staged through LDS: global loads and stores are
  * contiguous across the wave (lane t touches elements t, t+64, ...), each lane solves its
  * contiguous segment. */
-template <int SEG>
-__device__ __forceinline__ void xsolve_body(const XSolveArgs & A, double * __restrict__ dd,
-    const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
-    const double * __restrict__ lt, double * sm){
+template <int SEG, typename D>
+__device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict__ dd,
+    double * uout, const double * __restrict__ ucur, const double * __restrict__ rt,
+    const double * __restrict__ vt, const double * __restrict__ lt, double * sm){
   const tc_geom_t & g = A.g;
   const int w = threadIdx.x / TCK_WAVE;
   const int lane = threadIdx.x % TCK_WAVE;
@@ -370,7 +371,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, double * __res
       #pragma unroll
       for(int e = 0; e < SEG; e++){
         const int idx = lane + e * TCK_WAVE;
-        if(idx < n) sm[stage_pos<SEG>(idx)] = dd[base + idx];
+        if(idx < n) sm[stage_pos<SEG>(idx)] = double(dd[base + idx]);
       }
       if(ucur){
         #pragma unroll
@@ -393,13 +394,13 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, double * __res
         #pragma unroll
         for(int e = 0; e < SEG; e++){
           const int idx = lane + e * TCK_WAVE;
-          if(idx < n) tck_store_halo(dd, g, A.i0 + idx, j, k, u[e] + sm[stage_pos<SEG>(idx)], false);
+          if(idx < n) tck_store_halo(uout, g, A.i0 + idx, j, k, u[e] + sm[stage_pos<SEG>(idx)], false);
         }
       }else{
         #pragma unroll
         for(int e = 0; e < SEG; e++){
           const int idx = lane + e * TCK_WAVE;
-          if(idx < n) dd[base + idx] = sm[stage_pos<SEG>(idx)];
+          if(idx < n) dd[base + idx] = D(sm[stage_pos<SEG>(idx)]);
         }
       }
     }
@@ -407,10 +408,10 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, double * __res
   }
 }
 
-template <int SEG>
+template <int SEG, typename D>
 __global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
   __shared__ double xs_sm[TCK_BLOCK / TCK_WAVE][TCK_WAVE * (SEG + 1)];
-  xsolve_body<SEG>(A, A.d, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE]);
+  xsolve_body<SEG, D>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE]);
 }
 
 /* periodic lines in y (solve_in_y, :39-70) or z (solve_in_z, :72-103).  A block holds TI
@@ -429,7 +430,8 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
 
 struct PSolveArgs {
   tc_geom_t g;
-  double * d;
+  void * d;              /* the increment: double, or float in the mixed step */
+  double * uout;         /* last pass: the new field (the increment's buffer itself in FP64) */
   const double * ucur;
   const double * r; const double * v; const double * q1;
   const double * meta;  /* TC_PMETA rows [7][tstride] indexed by i (y) or [7] (z) */
@@ -463,7 +465,8 @@ __device__ __forceinline__ void ps_rows(double * out, const double * __restrict_
 
 /* body with restrict-qualified pointers: loads of the old field and the tables may be
  * scheduled ahead of the stores to the increment array */
-__device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __restrict__ d,
+template <typename D>
+__device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict__ d, double * uout,
     const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
     const double * __restrict__ q1t, const double * __restrict__ meta, double * ps_sm){
   const tc_geom_t & g = A.g;
@@ -501,8 +504,8 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
   double * smX = ps_sm + 2 * S * TI;     /* [3][TI]: q0[0], q0[n-2], q[n-1] */
   double q[PS_MAXM];
   #pragma unroll
-  for(int e = 0; e < PS_MAXM; e++) q[e] = (valid && e < cnt) ? d[base + step * (row0 + e)] : 0.;
-  if(valid && sg == S - 1) smX[2 * TI + ti] = d[base + step * m];
+  for(int e = 0; e < PS_MAXM; e++) q[e] = (valid && e < cnt) ? double(d[base + step * (row0 + e)]) : 0.;
+  if(valid && sg == S - 1) smX[2 * TI + ti] = double(d[base + step * m]);
   double t[PS_PRE];
   /* forward: q0 = q0 / c0, q_i = r_i (q_i - l q_{i-1}) */
   double a = 1., b = 0.;
@@ -611,9 +614,9 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
           if(ucur){
             const int jj = A.dir == 1 ? 1 + row : other;
             const int kk = A.dir == 1 ? other : 1 + row;
-            tck_store_halo(d, g, i, jj, kk, uu[e] + x, false);
+            tck_store_halo(uout, g, i, jj, kk, uu[e] + x, false);
           }else{
-            d[base + step * row] = x;
+            d[base + step * row] = D(x);
           }
         }
       }
@@ -623,16 +626,17 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, double * __res
     if(ucur){
       const int jj = A.dir == 1 ? n : other;
       const int kk = A.dir == 1 ? other : n;
-      tck_store_halo(d, g, i, jj, kk, ucur[base + step * m] + xl, false);
+      tck_store_halo(uout, g, i, jj, kk, ucur[base + step * m] + xl, false);
     }else{
-      d[base + step * m] = xl;
+      d[base + step * m] = D(xl);
     }
   }
 }
 
+template <typename D>
 __global__ void __launch_bounds__(512) tck_psolve(const PSolveArgs A){
   extern __shared__ double ps_sm[];
-  psolve_body(A, A.d, A.ucur, A.r, A.v, A.q1, A.meta, ps_sm);
+  psolve_body<D>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.q1, A.meta, ps_sm);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -642,6 +646,7 @@ __global__ void __launch_bounds__(512) tck_psolve(const PSolveArgs A){
  * ------------------------------------------------------------------------------------- */
 struct PFactorArgs {
   int nx, nky, nz;
+  int exact_gauge;   /* 1: the (0, 0) line takes x_{n-1} = 0 whatever its last pivot */
   const double * pl; const double * pu;     /* l_i, u_i (i = 0..nx-1) */
   const double * hyxc;                      /* HYXC(i), reference index */
   const double * evy; const double * evz;
@@ -677,19 +682,25 @@ __global__ void tck_pfactor(const PFactorArgs A){
   cl += A.pu[n - 1];
   const double val = cl - A.pl[n - 1] * vprev;
   r[n - 1] = fabs(val) > DBL_EPSILON ? 1. / val : 0.;
+  /* the Neumann operator of the (0, 0) mode is singular; rounding leaves its last pivot a few
+   * ulps from zero, so tdm.c's rule may divide by it.  That is harmless when the right-hand
+   * side sums to zero to FP64 rounding; an FP32 spectrum does not, and the division would
+   * swamp psi with a huge constant.  The mixed step fixes the gauge exactly instead. */
+  if(A.exact_gauge && line == 0) r[n - 1] = 0.;
   v[n - 1] = 0.;
 }
 
 struct PXSolveArgs {
   int nx, nlines;
-  double2 * s;               /* spectrum, line-major rows of nx complex */
+  void * s;                  /* spectrum, line-major rows of nx complex (double2, or float2
+                              * in the mixed step; the solve itself runs in FP64) */
   const double * pr; const double * pv; const double * pl;
 };
 
 /* one (ky, kz) line per wave per iteration, WPB waves per block looping over groups of WPB
  * lines; the spectrum row and its two factor rows are staged through LDS so that every
  * global access is contiguous across the wave (WPB keeps the staging area within 64 KB) */
-template <int SEG, int WPB>
+template <int SEG, int WPB, typename C2>
 __global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs A){
   __shared__ double2 smq_all[WPB][TCK_WAVE * (SEG + 1)];
   __shared__ double smr_all[WPB][TCK_WAVE * (SEG + 1)];
@@ -707,7 +718,7 @@ __global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs 
   for(long long g0 = (long long)blockIdx.x * WPB; g0 < A.nlines; g0 += (long long)gridDim.x * WPB){
     const long long line = g0 + w;
     const bool live = line < A.nlines;
-    double2 * row = A.s + (live ? line : 0) * n;
+    C2 * row = (C2 *)A.s + (live ? line : 0) * n;
     const double * r = A.pr + (live ? line : 0) * n;
     const double * v = A.pv + (live ? line : 0) * n;
     if(live){
@@ -716,7 +727,8 @@ __global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs 
         const int idx = lane + e * TCK_WAVE;
         if(idx < n){
           const int p = stage_pos<SEG>(idx);
-          smq[p] = row[idx];
+          const C2 x = row[idx];
+          smq[p] = make_double2(double(x.x), double(x.y));
           smr[p] = r[idx];
           smv[p] = v[idx];
         }
@@ -740,7 +752,11 @@ __global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs 
       #pragma unroll
       for(int e = 0; e < SEG; e++){
         const int idx = lane + e * TCK_WAVE;
-        if(idx < n) row[idx] = smq[stage_pos<SEG>(idx)];
+        if(idx < n){
+          const double2 x = smq[stage_pos<SEG>(idx)];
+          row[idx].x = x.x;
+          row[idx].y = x.y;
+        }
       }
     }
     __syncthreads();
@@ -755,24 +771,26 @@ static int seg_for(int n){
 }
 
 template <int SEG>
-static int launch_xsolve_t(const XSolveArgs & A, hipStream_t stream){
+static int launch_xsolve_t(const XSolveArgs & A, int mixed, hipStream_t stream){
   const long long nlines = (long long)A.g.ny * A.g.nz;
   const int wpb = TCK_BLOCK / TCK_WAVE;
   long long nblocks = (nlines + wpb - 1) / wpb;
   const long long cap = (long long)num_cus() * 8;
   if(nblocks > cap) nblocks = cap;
-  hipLaunchKernelGGL(tck_xsolve<SEG>, dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
+  if(mixed) hipLaunchKernelGGL((tck_xsolve<SEG, float>), dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
+  else hipLaunchKernelGGL((tck_xsolve<SEG, double>), dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
 
 template <int SEG>
-static int launch_pxsolve_t(const PXSolveArgs & A, hipStream_t stream){
+static int launch_pxsolve_t(const PXSolveArgs & A, int mixed, hipStream_t stream){
   constexpr int WPB = SEG >= 16 ? 1 : 2;
   long long nblocks = ((long long)A.nlines + WPB - 1) / WPB;
   const long long cap = (long long)num_cus() * 16 / WPB;
   if(nblocks > cap) nblocks = cap;
-  hipLaunchKernelGGL((tck_pxsolve<SEG, WPB>), dim3((unsigned)nblocks), dim3(WPB * TCK_WAVE), 0, stream, A);
+  if(mixed) hipLaunchKernelGGL((tck_pxsolve<SEG, WPB, float2>), dim3((unsigned)nblocks), dim3(WPB * TCK_WAVE), 0, stream, A);
+  else hipLaunchKernelGGL((tck_pxsolve<SEG, WPB, double2>), dim3((unsigned)nblocks), dim3(WPB * TCK_WAVE), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -809,25 +827,25 @@ int tck_launch_factor(const double * dt, const int dirs[3], int nx, int ny, int 
   return 0;
 }
 
-int tck_launch_xsolve(const tc_geom_t * g, double * d, const double * ucur, const double * r,
-    const double * v, const double * l, int i0, int n, hipStream_t stream){
+int tck_launch_xsolve(const tc_geom_t * g, void * d, double * uout, const double * ucur,
+    const double * r, const double * v, const double * l, int i0, int n, int mixed, hipStream_t stream){
   if(n > TCK_WAVE * XS_MAXSEG || n < 2) return (int)hipErrorInvalidValue;
   XSolveArgs A;
-  A.g = *g; A.d = d; A.ucur = ucur; A.r = r; A.v = v; A.l = l; A.i0 = i0; A.n = n;
+  A.g = *g; A.d = d; A.uout = uout; A.ucur = ucur; A.r = r; A.v = v; A.l = l; A.i0 = i0; A.n = n;
   switch(seg_for(n)){
-    case 1: return launch_xsolve_t<1>(A, stream);
-    case 2: return launch_xsolve_t<2>(A, stream);
-    case 4: return launch_xsolve_t<4>(A, stream);
-    case 8: return launch_xsolve_t<8>(A, stream);
-    default: return launch_xsolve_t<16>(A, stream);
+    case 1: return launch_xsolve_t<1>(A, mixed, stream);
+    case 2: return launch_xsolve_t<2>(A, mixed, stream);
+    case 4: return launch_xsolve_t<4>(A, mixed, stream);
+    case 8: return launch_xsolve_t<8>(A, mixed, stream);
+    default: return launch_xsolve_t<16>(A, mixed, stream);
   }
 }
 
-int tck_launch_psolve(const tc_geom_t * g, int dir, double * d, const double * ucur,
+int tck_launch_psolve(const tc_geom_t * g, int dir, void * d, double * uout, const double * ucur,
     const double * r, const double * v, const double * q1, const double * meta,
-    long long tstride, int tcol, int i0, int isz, hipStream_t stream){
+    long long tstride, int tcol, int i0, int isz, int mixed, hipStream_t stream){
   PSolveArgs A;
-  A.g = *g; A.d = d; A.ucur = ucur; A.r = r; A.v = v; A.q1 = q1; A.meta = meta;
+  A.g = *g; A.d = d; A.uout = uout; A.ucur = ucur; A.r = r; A.v = v; A.q1 = q1; A.meta = meta;
   A.tstride = tstride; A.tcol = tcol; A.dir = dir; A.i0 = i0; A.isz = isz;
   const int n = dir == 1 ? g->ny : g->nz;
   const int m = n - 1;
@@ -838,16 +856,17 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, double * d, const double * u
   const int nother = dir == 1 ? g->nz : g->ny;
   const size_t lds = sizeof(double) * (size_t)(2 * S * TI + 3 * TI);
   const int span = i0 + isz - 1;   /* tiles cover i = 1 .. i0+isz-1 */
-  hipLaunchKernelGGL(tck_psolve, dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
+  if(mixed) hipLaunchKernelGGL(tck_psolve<float>, dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
+  else hipLaunchKernelGGL(tck_psolve<double>, dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
 
 int tck_launch_pfactor(int nx, int nky, int nz, const double * pl, const double * pu,
     const double * hyxc, const double * evy, const double * evz, double hz, double * pr,
-    double * pv, hipStream_t stream){
+    double * pv, int exact_gauge, hipStream_t stream){
   PFactorArgs A;
-  A.nx = nx; A.nky = nky; A.nz = nz; A.pl = pl; A.pu = pu; A.hyxc = hyxc; A.evy = evy;
+  A.nx = nx; A.nky = nky; A.nz = nz; A.exact_gauge = exact_gauge; A.pl = pl; A.pu = pu; A.hyxc = hyxc; A.evy = evy;
   A.evz = evz; A.hz = hz; A.pr = pr; A.pv = pv;
   const long long nlines = (long long)nky * nz;
   hipLaunchKernelGGL(tck_pfactor, dim3((unsigned)((nlines + 63) / 64)), dim3(64), 0, stream, A);
@@ -855,17 +874,17 @@ int tck_launch_pfactor(int nx, int nky, int nz, const double * pl, const double 
   return 0;
 }
 
-int tck_launch_pxsolve(int nx, int nlines, double * s, const double * pr, const double * pv,
-    const double * pl, hipStream_t stream){
+int tck_launch_pxsolve(int nx, int nlines, void * s, const double * pr, const double * pv,
+    const double * pl, int mixed, hipStream_t stream){
   if(nx > TCK_WAVE * XS_MAXSEG || nx < 2) return (int)hipErrorInvalidValue;
   PXSolveArgs A;
-  A.nx = nx; A.nlines = nlines; A.s = (double2 *)s; A.pr = pr; A.pv = pv; A.pl = pl;
+  A.nx = nx; A.nlines = nlines; A.s = s; A.pr = pr; A.pv = pv; A.pl = pl;
   switch(seg_for(nx)){
-    case 1: return launch_pxsolve_t<1>(A, stream);
-    case 2: return launch_pxsolve_t<2>(A, stream);
-    case 4: return launch_pxsolve_t<4>(A, stream);
-    case 8: return launch_pxsolve_t<8>(A, stream);
-    default: return launch_pxsolve_t<16>(A, stream);
+    case 1: return launch_pxsolve_t<1>(A, mixed, stream);
+    case 2: return launch_pxsolve_t<2>(A, mixed, stream);
+    case 4: return launch_pxsolve_t<4>(A, mixed, stream);
+    case 8: return launch_pxsolve_t<8>(A, mixed, stream);
+    default: return launch_pxsolve_t<16>(A, mixed, stream);
   }
 }
 

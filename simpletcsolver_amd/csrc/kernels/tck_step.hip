@@ -56,9 +56,12 @@ struct PredictArgs {
   tc_coef_t c;
   tc_flags_t f;
   const double * ux; const double * uy; const double * uz; const double * t; const double * p;
-  double * nx_; double * ny_; double * nz_; double * nt;          /* increment or new field */
-  const double * sbx; const double * sby; const double * sbz; const double * sbt;   /* srcb */
-  double * sax; double * say; double * saz; double * sat;         /* srca (next substep's b) */
+  double * nx_; double * ny_; double * nz_; double * nt;          /* new field (final_here) */
+  /* buffers of the arithmetic type R of the instantiation (double, or float in the mixed
+   * step): the increment, srcb and srca (next substep's b) */
+  void * dx; void * dy; void * dz; void * dtt;
+  const void * sbx; const void * sby; const void * sbz; const void * sbt;
+  void * sax; void * say; void * saz; void * sat;
   const double * dt;
   int rkstep;
   int final_here;   /* 1: no line solve follows, write u + delta with halos */
@@ -75,14 +78,16 @@ struct PredictArgs {
 
 /* body with restrict-qualified pointers, so that the loads of each field section are free
  * to move ahead of the stores of the previous one */
+template <typename R>
 __device__ __forceinline__ void predict_body(const PredictArgs & A,
     const double * __restrict__ ux, const double * __restrict__ uy, const double * __restrict__ uz,
     const double * __restrict__ t, const double * __restrict__ p,
+    R * __restrict__ dxo, R * __restrict__ dyo, R * __restrict__ dzo, R * __restrict__ dto,
     double * __restrict__ nxo, double * __restrict__ nyo, double * __restrict__ nzo, double * __restrict__ nto,
-    const double * __restrict__ sbx, const double * __restrict__ sby, const double * __restrict__ sbz,
-    const double * __restrict__ sbt,
-    double * __restrict__ sax, double * __restrict__ say, double * __restrict__ saz, double * __restrict__ sat,
-    const int i, const int j, const int k, const long long kb0, const double * smt, const int ib){
+    const R * __restrict__ sbx, const R * __restrict__ sby, const R * __restrict__ sbz,
+    const R * __restrict__ sbt,
+    R * __restrict__ sax, R * __restrict__ say, R * __restrict__ saz, R * __restrict__ sat,
+    const int i, const int j, const int k, const long long kb0, const R * smt, const int ib){
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
   if(i > g.nx || j > g.ny) return;
@@ -91,296 +96,297 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   const int sy = g.px;
   const int sz = (int)g.plane;
   const int o = (i + TC_XOFF) + sy * j + sz;
-  const double nu = c.nu;
-  const double nu2 = 2. * nu;
-  const double dt = *A.dt;
+  const R nu = R(c.nu);
+  const R nu2 = R(2.) * nu;
+  const R dt = *A.dt;
   const int s = A.rkstep;
-  const double ca = c_rka[s] * dt;
-  const double cb = c_rkb[s] * dt;
-  const double cg = (c_rka[s] + c_rkb[s]) * dt;
+  const R ca = c_rka[s] * dt;
+  const R cb = c_rkb[s] * dt;
+  const R cg = (c_rka[s] + c_rkb[s]) * dt;
   const bool ix = A.f.implicit[0], iy = A.f.implicit[1], iz = A.f.implicit[2];
 
   /* ---------------- ux, faces i = 2..nx (predict/ux.c) ---------------- */
   if(i >= 2){
-    double a = 0., gg = 0.;
-    const double u0 = ux[o], uxm = ux[o - 1], uxp = ux[o + 1];
+    R a = R(0.), gg = R(0.);
+    const R u0 = R(ux[o]), uxm = R(ux[o - 1]), uxp = R(ux[o + 1]);
     { /* advection_x :92-124 */
-      const double ux_xm = CT(T_AXF, i - 1) * uxm + CT(T_AXF, i) * u0;
-      const double ux_xp = CT(T_AXF, i) * u0 + CT(T_AXF, i + 1) * uxp;
-      const double l = - 0.5 * ux_xm;
-      const double u = + 0.5 * ux_xp;
-      const double cc = - l - u;
+      const R ux_xm = CT(T_AXF, i - 1) * uxm + CT(T_AXF, i) * u0;
+      const R ux_xp = CT(T_AXF, i) * u0 + CT(T_AXF, i + 1) * uxp;
+      const R l = - R(0.5) * ux_xm;
+      const R u = + R(0.5) * ux_xp;
+      const R cc = - l - u;
       a -= CT(T_IJDXF, i) * (+ l * uxm + cc * u0 + u * uxp);
     }
     { /* advection_y :126-159 */
-      const double uy_ym = CT(T_AYC, i - 1) * uy[o - 1] + CT(T_AYC, i) * uy[o];
-      const double uy_yp = CT(T_AYC, i - 1) * uy[o - 1 + sy] + CT(T_AYC, i) * uy[o + sy];
-      const double l = - 0.5 * uy_ym;
-      const double u = + 0.5 * uy_yp;
-      const double cc = - l - u;
-      a -= CT(T_IJDXF, i) * (+ l * ux[o - sy] + cc * u0 + u * ux[o + sy]);
+      const R uy_ym = CT(T_AYC, i - 1) * R(uy[o - 1]) + CT(T_AYC, i) * R(uy[o]);
+      const R uy_yp = CT(T_AYC, i - 1) * R(uy[o - 1 + sy]) + CT(T_AYC, i) * R(uy[o + sy]);
+      const R l = - R(0.5) * uy_ym;
+      const R u = + R(0.5) * uy_yp;
+      const R cc = - l - u;
+      a -= CT(T_IJDXF, i) * (+ l * R(ux[o - sy]) + cc * u0 + u * R(ux[o + sy]));
     }
     { /* advection_z :161-192 */
-      const double uz_zm = CT(T_AZC, i - 1) * uz[o - 1] + CT(T_AZC, i) * uz[o];
-      const double uz_zp = CT(T_AZC, i - 1) * uz[o - 1 + sz] + CT(T_AZC, i) * uz[o + sz];
-      const double l = - 0.5 * uz_zm;
-      const double u = + 0.5 * uz_zp;
-      const double cc = - l - u;
-      a -= CT(T_IJDXF, i) * (+ l * ux[o - sz] + cc * u0 + u * ux[o + sz]);
+      const R uz_zm = CT(T_AZC, i - 1) * R(uz[o - 1]) + CT(T_AZC, i) * R(uz[o]);
+      const R uz_zp = CT(T_AZC, i - 1) * R(uz[o - 1 + sz]) + CT(T_AZC, i) * R(uz[o + sz]);
+      const R l = - R(0.5) * uz_zm;
+      const R u = + R(0.5) * uz_zp;
+      const R cc = - l - u;
+      a -= CT(T_IJDXF, i) * (+ l * R(ux[o - sz]) + cc * u0 + u * R(ux[o + sz]));
     }
     { /* advection_1 :194-226 (centrifugal) */
-      const double uy_xm = + 0.5 * uy[o - 1] + 0.5 * uy[o - 1 + sy];
-      const double uy_xp = + 0.5 * uy[o] + 0.5 * uy[o + sy];
+      const R uy_xm = + R(0.5) * R(uy[o - 1]) + R(0.5) * R(uy[o - 1 + sy]);
+      const R uy_xp = + R(0.5) * R(uy[o]) + R(0.5) * R(uy[o + sy]);
       a += CT(T_IJDXF, i) * (+ CT(T_D1M, i) * uy_xm * uy_xm + CT(T_D1P, i) * uy_xp * uy_xp);
     }
     { /* diffusion_x :228-248 */
-      const double v = nu2 * (+ CT(T_LXU_L, i) * uxm + CT(T_LXU_C, i) * u0 + CT(T_LXU_U, i) * uxp);
+      const R v = nu2 * (+ CT(T_LXU_L, i) * uxm + CT(T_LXU_C, i) * u0 + CT(T_LXU_U, i) * uxp);
       if(ix) gg += v; else a += v;
     }
     { /* diffusion_y0 :250-269 */
-      const double v = nu * (+ CT(T_LYU_L, i) * ux[o - sy] + CT(T_LYU_C, i) * u0 + CT(T_LYU_U, i) * ux[o + sy]);
+      const R v = nu * (+ CT(T_LYU_L, i) * R(ux[o - sy]) + CT(T_LYU_C, i) * u0 + CT(T_LYU_U, i) * R(ux[o + sy]));
       if(iy) gg += v; else a += v;
     }
     { /* diffusion_y1 :271-297; lyx1 (lyx.c:56-95) and lxy (lxy.c:19-49) at j and j+1 */
-      const double lyx_ym = CT(T_L1F, i) * (+ 0.5 * uy[o - 1] + 0.5 * uy[o]);
-      const double lyx_yp = CT(T_L1F, i) * (+ 0.5 * uy[o - 1 + sy] + 0.5 * uy[o + sy]);
-      const double lxy_ym = CT(T_IHXF, i) * (- uy[o - 1] + uy[o]);
-      const double lxy_yp = CT(T_IHXF, i) * (- uy[o - 1 + sy] + uy[o + sy]);
+      const R lyx_ym = CT(T_L1F, i) * (+ R(0.5) * R(uy[o - 1]) + R(0.5) * R(uy[o]));
+      const R lyx_yp = CT(T_L1F, i) * (+ R(0.5) * R(uy[o - 1 + sy]) + R(0.5) * R(uy[o + sy]));
+      const R lxy_ym = CT(T_IHXF, i) * (- R(uy[o - 1]) + R(uy[o]));
+      const R lxy_yp = CT(T_IHXF, i) * (- R(uy[o - 1 + sy]) + R(uy[o + sy]));
       a += CT(T_NUJF, i) * (- CT(T_JHYF, i) * (lyx_ym + lxy_ym) + CT(T_JHYF, i) * (lyx_yp + lxy_yp));
     }
     { /* diffusion_z0 :299-318 */
-      const double v = nu * (+ c.lapz_l * ux[o - sz] + c.lapz_c * u0 + c.lapz_u * ux[o + sz]);
+      const R v = nu * (+ R(c.lapz_l) * R(ux[o - sz]) + R(c.lapz_c) * u0 + R(c.lapz_u) * R(ux[o + sz]));
       if(iz) gg += v; else a += v;
     }
     { /* diffusion_z1 :320-342; lxz (lxz.c:10-44) at k and k+1 */
-      const double lxz_zm = CT(T_IHXF, i) * (- uz[o - 1] + uz[o]);
-      const double lxz_zp = CT(T_IHXF, i) * (- uz[o - 1 + sz] + uz[o + sz]);
+      const R lxz_zm = CT(T_IHXF, i) * (- R(uz[o - 1]) + R(uz[o]));
+      const R lxz_zp = CT(T_IHXF, i) * (- R(uz[o - 1 + sz]) + R(uz[o + sz]));
       a += CT(T_NUJF, i) * (- CT(T_JHZF, i) * lxz_zm + CT(T_JHZF, i) * lxz_zp);
     }
     { /* diffusion_a :344-375; lyy0 (lyy.c:24-54) + lyy1 (lyy.c:56-95) at i-1 and i */
-      const double lyy0_m = CT(T_IHYC, i - 1) * (- uy[o - 1] + uy[o - 1 + sy]);
-      const double lyy1_m = CT(T_L1C, i - 1) * (+ 0.5 * uxm + 0.5 * u0);
-      const double lyy0_p = CT(T_IHYC, i) * (- uy[o] + uy[o + sy]);
-      const double lyy1_p = CT(T_L1C, i) * (+ 0.5 * u0 + 0.5 * uxp);
-      const double lyy_xm = lyy0_m + lyy1_m;
-      const double lyy_xp = lyy0_p + lyy1_p;
+      const R lyy0_m = CT(T_IHYC, i - 1) * (- R(uy[o - 1]) + R(uy[o - 1 + sy]));
+      const R lyy1_m = CT(T_L1C, i - 1) * (+ R(0.5) * uxm + R(0.5) * u0);
+      const R lyy0_p = CT(T_IHYC, i) * (- R(uy[o]) + R(uy[o + sy]));
+      const R lyy1_p = CT(T_L1C, i) * (+ R(0.5) * u0 + R(0.5) * uxp);
+      const R lyy_xm = lyy0_m + lyy1_m;
+      const R lyy_xp = lyy0_p + lyy1_p;
       a -= CT(T_N2JF, i) * (+ CT(T_D1M, i) * lyy_xm + CT(T_D1P, i) * lyy_xp);
     }
     /* pressure :377-394 */
-    gg -= CT(T_IHXF, i) * (- p[o - 1] + p[o]);
-    double b = 0.;
+    gg -= CT(T_IHXF, i) * (- R(p[o - 1]) + R(p[o]));
+    R b = R(0.);
     if(sbx) b = sbx[o];
-    const double delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
+    const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sax) sax[o] = a;
-    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, u0 + delta, false);
-    else nxo[o] = delta;
+    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, ux[o] + (double)delta, false);
+    else dxo[o] = delta;
   }
 
   TCK_SECTION_FENCE();
   /* ---------------- uy, centres i = 1..nx (predict/uy.c) ---------------- */
   {
-    double a = 0., gg = 0.;
-    const double v0 = uy[o], vxm = uy[o - 1], vxp = uy[o + 1];
-    const double vym = uy[o - sy], vyp = uy[o + sy];
-    const double vzm = uy[o - sz], vzp = uy[o + sz];
+    R a = R(0.), gg = R(0.);
+    const R v0 = R(uy[o]), vxm = R(uy[o - 1]), vxp = R(uy[o + 1]);
+    const R vym = R(uy[o - sy]), vyp = R(uy[o + sy]);
+    const R vzm = R(uy[o - sz]), vzp = R(uy[o + sz]);
     /* ux at (i, j-1), (i, j), (i+1, j-1), (i+1, j) */
-    const double x00 = ux[o - sy], x01 = ux[o], x10 = ux[o + 1 - sy], x11 = ux[o + 1];
+    const R x00 = R(ux[o - sy]), x01 = R(ux[o]), x10 = R(ux[o + 1 - sy]), x11 = R(ux[o + 1]);
     { /* advection_x :93-126 */
-      const double ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
-      const double ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
-      const double l = - 0.5 * ux_xm;
-      const double u = + 0.5 * ux_xp;
-      const double cc = - l - u;
+      const R ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
+      const R ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
+      const R l = - R(0.5) * ux_xm;
+      const R u = + R(0.5) * ux_xp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * vxm + cc * v0 + u * vxp);
     }
     { /* advection_y :128-156 */
-      const double uy_ym = CT(T_AYC, i) * vym + CT(T_AYC, i) * v0;
-      const double uy_yp = CT(T_AYC, i) * v0 + CT(T_AYC, i) * vyp;
-      const double l = - 0.5 * uy_ym;
-      const double u = + 0.5 * uy_yp;
-      const double cc = - l - u;
+      const R uy_ym = CT(T_AYC, i) * vym + CT(T_AYC, i) * v0;
+      const R uy_yp = CT(T_AYC, i) * v0 + CT(T_AYC, i) * vyp;
+      const R l = - R(0.5) * uy_ym;
+      const R u = + R(0.5) * uy_yp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * vym + cc * v0 + u * vyp);
     }
     { /* advection_z :158-186 */
-      const double uz_zm = CT(T_AZC, i) * uz[o - sy] + CT(T_AZC, i) * uz[o];
-      const double uz_zp = CT(T_AZC, i) * uz[o - sy + sz] + CT(T_AZC, i) * uz[o + sz];
-      const double l = - 0.5 * uz_zm;
-      const double u = + 0.5 * uz_zp;
-      const double cc = - l - u;
+      const R uz_zm = CT(T_AZC, i) * R(uz[o - sy]) + CT(T_AZC, i) * R(uz[o]);
+      const R uz_zp = CT(T_AZC, i) * R(uz[o - sy + sz]) + CT(T_AZC, i) * R(uz[o + sz]);
+      const R l = - R(0.5) * uz_zm;
+      const R u = + R(0.5) * uz_zp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * vzm + cc * v0 + u * vzp);
     }
     { /* advection_1 :188-223 (Coriolis) */
-      const double uy_ym = + 0.5 * vym + 0.5 * v0;
-      const double uy_yp = + 0.5 * v0 + 0.5 * vyp;
-      const double ux_ym = + 0.5 * x00 + 0.5 * x10;
-      const double ux_yp = + 0.5 * x01 + 0.5 * x11;
+      const R uy_ym = + R(0.5) * vym + R(0.5) * v0;
+      const R uy_yp = + R(0.5) * v0 + R(0.5) * vyp;
+      const R ux_ym = + R(0.5) * x00 + R(0.5) * x10;
+      const R ux_yp = + R(0.5) * x01 + R(0.5) * x11;
       a -= CT(T_IJDXC, i) * (+ CT(T_HDC, i) * uy_ym * ux_ym + CT(T_HDC, i) * uy_yp * ux_yp);
     }
     { /* diffusion_x0 :225-244 */
-      const double v = nu * (+ CT(T_LXC_L, i) * vxm + CT(T_LXC_C, i) * v0 + CT(T_LXC_U, i) * vxp);
+      const R v = nu * (+ CT(T_LXC_L, i) * vxm + CT(T_LXC_C, i) * v0 + CT(T_LXC_U, i) * vxp);
       if(ix) gg += v; else a += v;
     }
     /* lyx0 (lyx.c:24-54), lyx1 (:56-95) and lxy (lxy.c:19-49) at faces i and i+1 */
-    const double lyx0_m = CT(T_IHYF, i) * (- x00 + x01);
-    const double lyx0_p = CT(T_IHYF, i + 1) * (- x10 + x11);
-    const double lyx1_m = CT(T_L1F, i) * (+ (1 == i ? 0. : 0.5) * vxm + 0.5 * v0);
-    const double lyx1_p = CT(T_L1F, i + 1) * (+ 0.5 * v0 + (g.nx == i ? 0. : 0.5) * vxp);
-    const double lyx_xm = lyx0_m + lyx1_m;
-    const double lyx_xp = lyx0_p + lyx1_p;
+    const R lyx0_m = CT(T_IHYF, i) * (- x00 + x01);
+    const R lyx0_p = CT(T_IHYF, i + 1) * (- x10 + x11);
+    const R lyx1_m = CT(T_L1F, i) * (+ (1 == i ? R(0.) : R(0.5)) * vxm + R(0.5) * v0);
+    const R lyx1_p = CT(T_L1F, i + 1) * (+ R(0.5) * v0 + (g.nx == i ? R(0.) : R(0.5)) * vxp);
+    const R lyx_xm = lyx0_m + lyx1_m;
+    const R lyx_xp = lyx0_p + lyx1_p;
     { /* diffusion_x1 :246-274 */
       a += CT(T_NUJC, i) * (- CT(T_JHXF, i) * lyx_xm + CT(T_JHXF, i + 1) * lyx_xp);
     }
     { /* diffusion_y0 :276-296 */
-      const double v = nu2 * (+ CT(T_LYC_L, i) * vym + CT(T_LYC_C, i) * v0 + CT(T_LYC_U, i) * vyp);
+      const R v = nu2 * (+ CT(T_LYC_L, i) * vym + CT(T_LYC_C, i) * v0 + CT(T_LYC_U, i) * vyp);
       if(iy) gg += v; else a += v;
     }
     { /* diffusion_y1 :298-320; lyy1 (lyy.c:56-95) at j-1 and j */
-      const double lyy1_m = CT(T_L1C, i) * (+ 0.5 * x00 + 0.5 * x10);
-      const double lyy1_p = CT(T_L1C, i) * (+ 0.5 * x01 + 0.5 * x11);
+      const R lyy1_m = CT(T_L1C, i) * (+ R(0.5) * x00 + R(0.5) * x10);
+      const R lyy1_p = CT(T_L1C, i) * (+ R(0.5) * x01 + R(0.5) * x11);
       a += CT(T_N2JC, i) * (- CT(T_JHYC, i) * lyy1_m + CT(T_JHYC, i) * lyy1_p);
     }
     { /* diffusion_z0 :322-341 */
-      const double v = nu * (+ c.lapz_l * vzm + c.lapz_c * v0 + c.lapz_u * vzp);
+      const R v = nu * (+ R(c.lapz_l) * vzm + R(c.lapz_c) * v0 + R(c.lapz_u) * vzp);
       if(iz) gg += v; else a += v;
     }
     { /* diffusion_z1 :343-365; lyz (lyz.c:10-44) at k and k+1 */
-      const double lyz_zm = CT(T_IHYC, i) * (- uz[o - sy] + uz[o]);
-      const double lyz_zp = CT(T_IHYC, i) * (- uz[o - sy + sz] + uz[o + sz]);
+      const R lyz_zm = CT(T_IHYC, i) * (- R(uz[o - sy]) + R(uz[o]));
+      const R lyz_zp = CT(T_IHYC, i) * (- R(uz[o - sy + sz]) + R(uz[o + sz]));
       a += CT(T_NUJC, i) * (- CT(T_JHZC, i) * lyz_zm + CT(T_JHZC, i) * lyz_zp);
     }
     { /* diffusion_a :367-400 */
-      const double lxy_xm = CT(T_IHXF, i) * (- vxm + v0);
-      const double lxy_xp = CT(T_IHXF, i + 1) * (- v0 + vxp);
+      const R lxy_xm = CT(T_IHXF, i) * (- vxm + v0);
+      const R lxy_xp = CT(T_IHXF, i + 1) * (- v0 + vxp);
       a += CT(T_NUJC, i) * (+ CT(T_DAM, i) * (lyx_xm + lxy_xm) + CT(T_DAP, i) * (lyx_xp + lxy_xp));
     }
     /* pressure :402-419 */
-    gg -= CT(T_IHYC, i) * (- p[o - sy] + p[o]);
-    double b = 0.;
+    gg -= CT(T_IHYC, i) * (- R(p[o - sy]) + R(p[o]));
+    R b = R(0.);
     if(sby) b = sby[o];
-    const double delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
+    const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(say) say[o] = a;
-    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, v0 + delta, false);
-    else nyo[o] = delta;
+    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, uy[o] + (double)delta, false);
+    else dyo[o] = delta;
   }
 
   TCK_SECTION_FENCE();
   /* ---------------- uz, centres (predict/uz.c) ---------------- */
   {
-    double a = 0., gg = 0.;
-    const double w0 = uz[o], wxm = uz[o - 1], wxp = uz[o + 1];
-    const double wym = uz[o - sy], wyp = uz[o + sy];
-    const double wzm = uz[o - sz], wzp = uz[o + sz];
+    R a = R(0.), gg = R(0.);
+    const R w0 = R(uz[o]), wxm = R(uz[o - 1]), wxp = R(uz[o + 1]);
+    const R wym = R(uz[o - sy]), wyp = R(uz[o + sy]);
+    const R wzm = R(uz[o - sz]), wzp = R(uz[o + sz]);
     /* ux at (i, k-1), (i, k), (i+1, k-1), (i+1, k) */
-    const double x00 = ux[o - sz], x01 = ux[o], x10 = ux[o + 1 - sz], x11 = ux[o + 1];
+    const R x00 = R(ux[o - sz]), x01 = R(ux[o]), x10 = R(ux[o + 1 - sz]), x11 = R(ux[o + 1]);
     /* uy at (j, k-1), (j, k), (j+1, k-1), (j+1, k) */
-    const double y00 = uy[o - sz], y01 = uy[o], y10 = uy[o + sy - sz], y11 = uy[o + sy];
+    const R y00 = R(uy[o - sz]), y01 = R(uy[o]), y10 = R(uy[o + sy - sz]), y11 = R(uy[o + sy]);
     { /* advection_x :88-121 */
-      const double ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
-      const double ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
-      const double l = - 0.5 * ux_xm;
-      const double u = + 0.5 * ux_xp;
-      const double cc = - l - u;
+      const R ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
+      const R ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
+      const R l = - R(0.5) * ux_xm;
+      const R u = + R(0.5) * ux_xp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * wxm + cc * w0 + u * wxp);
     }
     { /* advection_y :123-152 */
-      const double uy_ym = CT(T_AYC, i) * y00 + CT(T_AYC, i) * y01;
-      const double uy_yp = CT(T_AYC, i) * y10 + CT(T_AYC, i) * y11;
-      const double l = - 0.5 * uy_ym;
-      const double u = + 0.5 * uy_yp;
-      const double cc = - l - u;
+      const R uy_ym = CT(T_AYC, i) * y00 + CT(T_AYC, i) * y01;
+      const R uy_yp = CT(T_AYC, i) * y10 + CT(T_AYC, i) * y11;
+      const R l = - R(0.5) * uy_ym;
+      const R u = + R(0.5) * uy_yp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * wym + cc * w0 + u * wyp);
     }
     { /* advection_z :154-181 */
-      const double uz_zm = CT(T_AZC, i) * wzm + CT(T_AZC, i) * w0;
-      const double uz_zp = CT(T_AZC, i) * w0 + CT(T_AZC, i) * wzp;
-      const double l = - 0.5 * uz_zm;
-      const double u = + 0.5 * uz_zp;
-      const double cc = - l - u;
+      const R uz_zm = CT(T_AZC, i) * wzm + CT(T_AZC, i) * w0;
+      const R uz_zp = CT(T_AZC, i) * w0 + CT(T_AZC, i) * wzp;
+      const R l = - R(0.5) * uz_zm;
+      const R u = + R(0.5) * uz_zp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * wzm + cc * w0 + u * wzp);
     }
     { /* diffusion_x0 :183-202 */
-      const double v = nu * (+ CT(T_LXC_L, i) * wxm + CT(T_LXC_C, i) * w0 + CT(T_LXC_U, i) * wxp);
+      const R v = nu * (+ CT(T_LXC_L, i) * wxm + CT(T_LXC_C, i) * w0 + CT(T_LXC_U, i) * wxp);
       if(ix) gg += v; else a += v;
     }
     { /* diffusion_x1 :204-231; lzx (lzx.c:9-42) at faces i and i+1 */
-      const double lzx_xm = c.ihz * (- x00 + x01);
-      const double lzx_xp = c.ihz * (- x10 + x11);
+      const R lzx_xm = R(c.ihz) * (- x00 + x01);
+      const R lzx_xp = R(c.ihz) * (- x10 + x11);
       a += CT(T_NUJC, i) * (- CT(T_JHXF, i) * lzx_xm + CT(T_JHXF, i + 1) * lzx_xp);
     }
     { /* diffusion_y0 :233-252 */
-      const double v = nu * (+ CT(T_LYC_L, i) * wym + CT(T_LYC_C, i) * w0 + CT(T_LYC_U, i) * wyp);
+      const R v = nu * (+ CT(T_LYC_L, i) * wym + CT(T_LYC_C, i) * w0 + CT(T_LYC_U, i) * wyp);
       if(iy) gg += v; else a += v;
     }
     { /* diffusion_y1 :254-277; lzy (lzy.c:9-42) at j and j+1 */
-      const double lzy_ym = c.ihz * (- y00 + y01);
-      const double lzy_yp = c.ihz * (- y10 + y11);
+      const R lzy_ym = R(c.ihz) * (- y00 + y01);
+      const R lzy_yp = R(c.ihz) * (- y10 + y11);
       a += CT(T_NUJC, i) * (- CT(T_JHYC, i) * lzy_ym + CT(T_JHYC, i) * lzy_yp);
     }
     { /* diffusion_z :279-299 */
-      const double v = nu2 * (+ c.lapz_l * wzm + c.lapz_c * w0 + c.lapz_u * wzp);
+      const R v = nu2 * (+ R(c.lapz_l) * wzm + R(c.lapz_c) * w0 + R(c.lapz_u) * wzp);
       if(iz) gg += v; else a += v;
     }
     /* pressure :301-318 */
-    gg -= c.ihz * (- p[o - sz] + p[o]);
-    double b = 0.;
+    gg -= R(c.ihz) * (- R(p[o - sz]) + R(p[o]));
+    R b = R(0.);
     if(sbz) b = sbz[o];
-    const double delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
+    const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(saz) saz[o] = a;
-    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, w0 + delta, false);
-    else nzo[o] = delta;
+    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, uz[o] + (double)delta, false);
+    else dzo[o] = delta;
   }
 
   TCK_SECTION_FENCE();
   /* ---------------- T, centres (predict/t.c) ---------------- */
   if(A.f.scalar){
-    const double kappa = c.kappa;
-    double a = 0., gg = 0.;
-    const double t0 = t[o], txm = t[o - 1], txp = t[o + 1];
-    const double tym = t[o - sy], typ = t[o + sy];
-    const double tzm = t[o - sz], tzp = t[o + sz];
+    const R kappa = R(c.kappa);
+    R a = R(0.), gg = R(0.);
+    const R t0 = R(t[o]), txm = R(t[o - 1]), txp = R(t[o + 1]);
+    const R tym = R(t[o - sy]), typ = R(t[o + sy]);
+    const R tzm = R(t[o - sz]), tzp = R(t[o + sz]);
     { /* advection_x :88-119 */
-      const double ux_xm = CT(T_JHXF, i) * ux[o];
-      const double ux_xp = CT(T_JHXF, i + 1) * ux[o + 1];
-      const double l = - 0.5 * ux_xm;
-      const double u = + 0.5 * ux_xp;
-      const double cc = - l - u;
+      const R ux_xm = CT(T_JHXF, i) * R(ux[o]);
+      const R ux_xp = CT(T_JHXF, i + 1) * R(ux[o + 1]);
+      const R l = - R(0.5) * ux_xm;
+      const R u = + R(0.5) * ux_xp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * txm + cc * t0 + u * txp);
     }
     { /* advection_y :121-148 */
-      const double uy_ym = CT(T_JHYC, i) * uy[o];
-      const double uy_yp = CT(T_JHYC, i) * uy[o + sy];
-      const double l = - 0.5 * uy_ym;
-      const double u = + 0.5 * uy_yp;
-      const double cc = - l - u;
+      const R uy_ym = CT(T_JHYC, i) * R(uy[o]);
+      const R uy_yp = CT(T_JHYC, i) * R(uy[o + sy]);
+      const R l = - R(0.5) * uy_ym;
+      const R u = + R(0.5) * uy_yp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * tym + cc * t0 + u * typ);
     }
     { /* advection_z :150-176 */
-      const double uz_zm = CT(T_JHZC, i) * uz[o];
-      const double uz_zp = CT(T_JHZC, i) * uz[o + sz];
-      const double l = - 0.5 * uz_zm;
-      const double u = + 0.5 * uz_zp;
-      const double cc = - l - u;
+      const R uz_zm = CT(T_JHZC, i) * R(uz[o]);
+      const R uz_zp = CT(T_JHZC, i) * R(uz[o + sz]);
+      const R l = - R(0.5) * uz_zm;
+      const R u = + R(0.5) * uz_zp;
+      const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * tzm + cc * t0 + u * tzp);
     }
     { /* diffusion_x/y/z :178-239 */
-      const double vx = kappa * (+ CT(T_LXC_L, i) * txm + CT(T_LXC_C, i) * t0 + CT(T_LXC_U, i) * txp);
+      const R vx = kappa * (+ CT(T_LXC_L, i) * txm + CT(T_LXC_C, i) * t0 + CT(T_LXC_U, i) * txp);
       if(ix) gg += vx; else a += vx;
-      const double vy = kappa * (+ CT(T_LYC_L, i) * tym + CT(T_LYC_C, i) * t0 + CT(T_LYC_U, i) * typ);
+      const R vy = kappa * (+ CT(T_LYC_L, i) * tym + CT(T_LYC_C, i) * t0 + CT(T_LYC_U, i) * typ);
       if(iy) gg += vy; else a += vy;
-      const double vz = kappa * (+ c.lapz_l * tzm + c.lapz_c * t0 + c.lapz_u * tzp);
+      const R vz = kappa * (+ R(c.lapz_l) * tzm + R(c.lapz_c) * t0 + R(c.lapz_u) * tzp);
       if(iz) gg += vz; else a += vz;
     }
-    double b = 0.;
+    R b = R(0.);
     if(sbt) b = sbt[o];
-    const double delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
+    const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sat) sat[o] = a;
-    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, t0 + delta, false);
-    else nto[o] = delta;
+    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, t[o] + (double)delta, false);
+    else dto[o] = delta;
   }
 }
 
+template <typename R>
 __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) tck_predict(const PredictArgs A){
-  __shared__ double smt[T_NPRED * TCK_TW];
+  __shared__ R smt[T_NPRED * TCK_TW];
   int bx, by, bz;
   tile_of(A.tm, &bx, &by, &bz);
   const int ib = bx * TCK_WAVE;           /* i of LDS column 0: the tile starts at i = ib + 1 */
@@ -389,7 +395,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
   for(int q = tid; q < T_NPRED * TCK_TW; q += nth){
     const int t = q / TCK_TW;
     const int ii = ib + q % TCK_TW;
-    smt[q] = ii < A.c.tp ? A.c.tab[(long long)t * A.c.tp + ii] : 0.;
+    smt[q] = ii < A.c.tp ? R(A.c.tab[(long long)t * A.c.tp + ii]) : R(0.);
   }
   __syncthreads();
   const int i = 1 + ib + threadIdx.x;
@@ -399,11 +405,16 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
   for(int k = k0; k < k1; k++){
     const long long kb0 = (long long)(k - 1) * A.g.plane;
     #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
-    predict_body(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, RB(A.t), A.p + kb0,
-        A.nx_ + kb0, A.ny_ + kb0, A.nz_ + kb0, RB(A.nt),
-        RB(A.sbx), RB(A.sby), RB(A.sbz), RB(A.sbt), RB(A.sax), RB(A.say), RB(A.saz), RB(A.sat),
+    #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
+    #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
+    predict_body<R>(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, RB(A.t), A.p + kb0,
+        RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
+        RB(A.nx_), RB(A.ny_), RB(A.nz_), RB(A.nt),
+        RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat),
         i, j, k, kb0, smt, ib);
     #undef RB
+    #undef RBT
+    #undef RBC
   }
 }
 
@@ -414,7 +425,7 @@ struct CorrectArgs {
   tc_coef_t c;
   tc_flags_t f;
   double * ux; double * uy; double * uz; double * p;
-  const double * psi;
+  const void * psi;             /* the potential: double, or float in the mixed step */
   const double * dt;
   int rkstep;
   int mode;                     /* bit 0: velocity correction, bit 1: pressure update */
@@ -422,9 +433,10 @@ struct CorrectArgs {
   TileMap tm;
 };
 
+template <typename R>
 __device__ __forceinline__ double correct_body(const CorrectArgs & A, double * __restrict__ ux,
     double * __restrict__ uy, double * __restrict__ uz, double * __restrict__ pp,
-    const double * __restrict__ psi, const int i, const int j, const int k, const long long kb0){
+    const R * __restrict__ psi, const int i, const int j, const int k, const long long kb0){
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
   const bool inside = (i <= g.nx && j <= g.ny);
@@ -439,22 +451,22 @@ __device__ __forceinline__ double correct_body(const CorrectArgs & A, double * _
     const double gamma = c_rka[s] + c_rkb[s];
     const double prefactor = gamma * dt;                /* correct/main.c:22-23 */
     const double small = 1.e-8;
-    const double ps0 = psi[o];
+    const double ps0 = double(psi[o]);
     if(A.mode & 1){
     if(i >= 2){ /* correct/ux.c:25-52 */
       const double hx = c.hxxf[i];
-      const double v = ux[o] - prefactor / hx * (- psi[o - 1] + ps0);
+      const double v = ux[o] - prefactor / hx * (- double(psi[o - 1]) + ps0);
       tck_store_halo(ux - kb0, g, i, j, k, v, false);
       lmin = fmin(lmin, hx / (fabs(v) + small));
     }
     { /* correct/uy.c:25-52 */
       const double hy = c.hyxc[i];
-      const double v = uy[o] - prefactor / hy * (- psi[o - sy] + ps0);
+      const double v = uy[o] - prefactor / hy * (- double(psi[o - sy]) + ps0);
       tck_store_halo(uy - kb0, g, i, j, k, v, false);
       lmin = fmin(lmin, hy / (fabs(v) + small));
     }
     { /* correct/uz.c:15-44 */
-      const double v = uz[o] - prefactor / c.hz * (- psi[o - sz] + ps0);
+      const double v = uz[o] - prefactor / c.hz * (- double(psi[o - sz]) + ps0);
       tck_store_halo(uz - kb0, g, i, j, k, v, false);
       lmin = fmin(lmin, c.hz / (fabs(v) + small));
     }
@@ -466,19 +478,19 @@ __device__ __forceinline__ double correct_body(const CorrectArgs & A, double * _
         const double l = c.plx_l[i];
         const double u = c.plx_u[i];
         const double cc = - l - u;
-        v -= 2. * pf / c.jdxc[i] * (+ l * psi[o - 1] + cc * ps0 + u * psi[o + 1]);
+        v -= 2. * pf / c.jdxc[i] * (+ l * double(psi[o - 1]) + cc * ps0 + u * double(psi[o + 1]));
       }
       if(A.f.implicit[1]){
         const double l = c.ply[i];
         const double u = c.ply[i];
         const double cc = - l - u;
-        v -= 2. * pf / c.jdxc[i] * (+ l * psi[o - sy] + cc * ps0 + u * psi[o + sy]);
+        v -= 2. * pf / c.jdxc[i] * (+ l * double(psi[o - sy]) + cc * ps0 + u * double(psi[o + sy]));
       }
       if(A.f.implicit[2]){
         const double l = c.plz[i];
         const double u = c.plz[i];
         const double cc = - l - u;
-        v -= 2. * pf / c.jdxc[i] * (+ l * psi[o - sz] + cc * ps0 + u * psi[o + sz]);
+        v -= 2. * pf / c.jdxc[i] * (+ l * double(psi[o - sz]) + cc * ps0 + u * double(psi[o + sz]));
       }
       tck_store_halo(pp - kb0, g, i, j, k, v, true);
     }
@@ -501,6 +513,7 @@ __device__ __forceinline__ void dt_block_min(double v, unsigned long long * dtmi
   }
 }
 
+template <typename R>
 __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
   int bx, by, bz;
   tile_of(A.tm, &bx, &by, &bz);
@@ -513,7 +526,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
     int il = i;
     asm volatile("" : "+v"(il));
     const long long kb0 = (long long)(k - 1) * A.g.plane;
-    lmin = fmin(lmin, correct_body(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, A.p + kb0, A.psi + kb0, il, j, k, kb0));
+    lmin = fmin(lmin, correct_body<R>(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, A.p + kb0, (const R *)A.psi + kb0, il, j, k, kb0));
   }
   /* CFL bound of decide_dt_adv (decide_dt.c:32-80) on the corrected velocity */
   if(A.dtmin) dt_block_min(lmin, A.dtmin);
@@ -593,26 +606,28 @@ static int check_shape(int kb, int ty){
 }
 
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
-    const double * const cur[4], const double * p, double * const nxt[4],
-    const double * const srcb[4], double * const srca[4], const double * dt, int rkstep,
-    int final_here, int kb, int ty, hipStream_t stream){
+    const double * const cur[4], const double * p, double * const nxt[4], void * const dlt[4],
+    const void * const srcb[4], void * const srca[4], const double * dt, int rkstep,
+    int final_here, int mixed, int kb, int ty, hipStream_t stream){
   if(check_shape(kb, ty)) return (int)hipErrorInvalidValue;
   PredictArgs A;
   A.g = *g; A.c = *c; A.f = *f;
   A.ux = cur[0]; A.uy = cur[1]; A.uz = cur[2]; A.t = cur[3]; A.p = p;
   A.nx_ = nxt[0]; A.ny_ = nxt[1]; A.nz_ = nxt[2]; A.nt = nxt[3];
+  A.dx = dlt[0]; A.dy = dlt[1]; A.dz = dlt[2]; A.dtt = dlt[3];
   A.sbx = srcb[0]; A.sby = srcb[1]; A.sbz = srcb[2]; A.sbt = srcb[3];
   A.sax = srca[0]; A.say = srca[1]; A.saz = srca[2]; A.sat = srca[3];
   A.dt = dt; A.rkstep = rkstep; A.final_here = final_here;
   A.tm = cell_tiles(*g, kb, ty);
-  hipLaunchKernelGGL(tck_predict, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
+  if(mixed) hipLaunchKernelGGL(tck_predict<float>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
+  else hipLaunchKernelGGL(tck_predict<double>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
 
 int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
-    double * ux, double * uy, double * uz, double * p, const double * psi, const double * dt,
-    int rkstep, int mode, unsigned long long * dtmin, int kb, int ty, hipStream_t stream){
+    double * ux, double * uy, double * uz, double * p, const void * psi, const double * dt,
+    int rkstep, int mode, unsigned long long * dtmin, int mixed, int kb, int ty, hipStream_t stream){
   if(check_shape(kb, ty)) return (int)hipErrorInvalidValue;
   CorrectArgs A;
   A.mode = mode;
@@ -620,7 +635,8 @@ int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
   A.ux = ux; A.uy = uy; A.uz = uz; A.p = p; A.psi = psi;
   A.dt = dt; A.rkstep = rkstep; A.dtmin = dtmin;
   A.tm = cell_tiles(*g, kb, ty);
-  hipLaunchKernelGGL(tck_correct, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
+  if(mixed) hipLaunchKernelGGL(tck_correct<float>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
+  else hipLaunchKernelGGL(tck_correct<double>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
