@@ -47,6 +47,7 @@ typedef struct {
 } tc_params_t;
 
 typedef struct tc_solver_s tc_solver_t;
+typedef struct tc_cnn_s tc_cnn_t;
 
 /* defaults of the reference (src/param/*.c), Re = Pr = 1, coefficients 1 */
 void tc_params_default(tc_params_t * p);
@@ -111,6 +112,51 @@ int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num)
  * marched per block (1..nz) and ty j rows per block of 64 x ty threads (1, 2 or 4).  Results
  * are identical for every shape; only the speed changes */
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty);
+/* no reference counterpart: times the predictor for every candidate shape (kb in 2..32,
+ * ty in 1, 2, 4) on the solver's own buffers, reps launches each, and keeps the fastest for
+ * the predictor and the correction.  Only the next-field and RK buffers are written, which
+ * the next step overwrites, so the flow itself is unchanged. */
+int tc_autotune(tc_solver_t * s, int reps, int * kb, int * ty, double * best_ms);
+
+/* no reference counterpart: n independent flows (any sizes, parameters and precisions, one
+ * device) advanced together.  The members share one stream; the first tc_ensemble_step takes
+ * one eager step and records each RK substep of the whole batch into a hipGraph (for both
+ * parities of the members' double buffers), every later step is three graph launches.
+ * Members are ordinary solvers for set/get/diagnostics (tc_ensemble_member); after a member's
+ * fields are replaced, or after it is stepped on its own, call tc_ensemble_invalidate. */
+typedef struct tc_ensemble_s tc_ensemble_t;
+int tc_ensemble_create(int n, const tc_params_t * params, const double * const * xf,
+    const double * const * xc, tc_ensemble_t ** out);
+tc_solver_t * tc_ensemble_member(tc_ensemble_t * e, int m);
+/* tc_autotune on every member (members of the same shape share the result); kb, ty: optional
+ * outputs of n entries */
+int tc_ensemble_autotune(tc_ensemble_t * e, int reps, int * kb, int * ty);
+int tc_ensemble_step(tc_ensemble_t * e, int nsteps);
+int tc_ensemble_synchronize(tc_ensemble_t * e);
+int tc_ensemble_invalidate(tc_ensemble_t * e);
+int tc_ensemble_destroy(tc_ensemble_t * e);
+
+/* no reference counterpart: the learned subgrid closure.  A 3-D CNN on a grid of
+ * nx x ny x nz cells (3x3x3 convolutions 5 -> 32 -> 32 -> 4 channels, ReLU, BF16 operands on
+ * the matrix cores with FP32 accumulation; x zero-padded, y and z periodic), trained on the
+ * device: inputs x [batch][nz][ny][nx][5] (u, v, w, T, r at cell centres, normalised by the
+ * caller), targets y [batch][nz][ny][nx][4] (corrections of the ux, uy, uz, T right-hand
+ * sides), mean-square loss, Adam.  Layer l weights are [27][co][ci] with co, ci padded to
+ * (32, 16), (32, 32), (16, 32); tap = dx+1 + 3 (dy+1) + 9 (dz+1). */
+int tc_cnn_create(int nx, int ny, int nz, int batch, unsigned long long seed, int device, tc_cnn_t ** out);
+int tc_cnn_destroy(tc_cnn_t * c);
+int tc_cnn_set_layer(tc_cnn_t * c, int layer, const float * w, const float * b);
+/* weights, biases and the gradients of the last tc_cnn_gradients / tc_cnn_train_step */
+int tc_cnn_get_layer(tc_cnn_t * c, int layer, float * w, float * b, float * gw, float * gb);
+int tc_cnn_predict(tc_cnn_t * c, const float * x, float * y);
+int tc_cnn_gradients(tc_cnn_t * c, const float * x, const float * y, double * loss);
+int tc_cnn_train_step(tc_cnn_t * c, const float * x, const float * y, float lr, double * loss);
+/* couple a batch-1 network of the solver's grid (null: detach): at the start of every step
+ * the inputs are formed from the current fields ((q - in_mean) / in_std), the network runs on
+ * the solver's stream, and out_scale times its outputs are added to the explicit right-hand
+ * sides (ux at faces as the mean of the two neighbouring centres) in every substep */
+int tc_closure_attach(tc_solver_t * s, tc_cnn_t * c, const float in_mean[5], const float in_std[5],
+    const float out_scale[4]);
 
 /* events around the kernels of one named stage ("predict", "poisson", ...) while timing */
 int tc_profile_enable(tc_solver_t * s, int on);

@@ -1,0 +1,180 @@
+/*
+ * tc_ensemble.c -- a batch of independent flows of uneven sizes advanced together.
+ *
+ * The reference runs one flow per process.  Here n solvers share one HIP stream; after one
+ * eager step, each RK substep of the whole batch (dt and factor tables in the first, then
+ * predict, Poisson solve and correction of every member) is recorded once into a hipGraph, so
+ * a step of the batch is three graph launches whatever n is.  The members' double buffers
+ * alternate from step to step, so the substeps of two consecutive steps are recorded (six
+ * graphs) and replayed by step parity; the members' host-side bookkeeping advances with each
+ * replayed step exactly as an eager step would advance it.
+ *
+ * Launch shapes: tc_ensemble_autotune times the predictor of every member over the candidate
+ * tiles and keeps the fastest, per member (members of the same shape reuse the first result).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <hip/hip_runtime_api.h>
+#include "tcstep.h"
+#include "tc_internal.h"
+
+struct tc_ensemble_s {
+  int n;
+  tc_solver_t ** s;
+  hipStream_t stream;
+  int device;
+  int recorded;
+  long long steps;            /* steps taken since the recording (parity of the replay) */
+  hipGraph_t graph[2][3];
+  hipGraphExec_t exec[2][3];
+};
+
+int tc_ensemble_create(int n, const tc_params_t * params, const double * const * xf,
+    const double * const * xc, tc_ensemble_t ** out){
+  *out = NULL;
+  if(n < 1) return tc_fail("tc_ensemble_create: n = %d", n);
+  tc_ensemble_t * e = calloc(1, sizeof(*e));
+  if(!e) return tc_fail("host allocation failed");
+  e->s = calloc((size_t)n, sizeof(tc_solver_t *));
+  if(!e->s){ free(e); return tc_fail("host allocation failed"); }
+  e->device = params[0].device;
+  if(hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess){
+    free(e->s);
+    free(e);
+    return tc_fail("tc_ensemble_create: stream creation failed");
+  }
+  for(int m = 0; m < n; m++){
+    if(params[m].device != e->device){
+      tc_ensemble_destroy(e);
+      return tc_fail("tc_ensemble_create: all members must use device %d", e->device);
+    }
+    if(tc_create(&params[m], xf[m], xc[m], &e->s[m]) || tc_share_stream(e->s[m], e->stream)){
+      tc_ensemble_destroy(e);
+      return 1;
+    }
+    e->n = m + 1;
+  }
+  *out = e;
+  return 0;
+}
+
+tc_solver_t * tc_ensemble_member(tc_ensemble_t * e, int m){
+  if(!e || m < 0 || m >= e->n) return NULL;
+  return e->s[m];
+}
+
+static void drop_graphs(tc_ensemble_t * e){
+  for(int p = 0; p < 2; p++){
+    for(int r = 0; r < 3; r++){
+      if(e->exec[p][r]) hipGraphExecDestroy(e->exec[p][r]);
+      if(e->graph[p][r]) hipGraphDestroy(e->graph[p][r]);
+      e->exec[p][r] = NULL;
+      e->graph[p][r] = NULL;
+    }
+  }
+  e->recorded = 0;
+}
+
+int tc_ensemble_destroy(tc_ensemble_t * e){
+  if(!e) return 0;
+  hipSetDevice(e->device);
+  if(e->stream) hipStreamSynchronize(e->stream);
+  drop_graphs(e);
+  for(int m = 0; m < e->n; m++) tc_destroy(e->s[m]);
+  if(e->stream) hipStreamDestroy(e->stream);
+  free(e->s);
+  free(e);
+  return 0;
+}
+
+int tc_ensemble_autotune(tc_ensemble_t * e, int reps, int * kb, int * ty){
+  HIPOK(hipSetDevice(e->device));
+  for(int m = 0; m < e->n; m++){
+    const tc_params_t * p = &e->s[m]->p;
+    int done = 0;
+    for(int q = 0; q < m && !done; q++){
+      const tc_params_t * pq = &e->s[q]->p;
+      if(pq->nx == p->nx && pq->ny == p->ny && pq->nz == p->nz && pq->precision == p->precision &&
+         pq->scalar == p->scalar && pq->implicit[0] == p->implicit[0] && pq->implicit[1] == p->implicit[1] &&
+         pq->implicit[2] == p->implicit[2]){
+        memcpy(e->s[m]->tune, e->s[q]->tune, sizeof(e->s[m]->tune));
+        done = 1;
+      }
+    }
+    if(!done && tc_autotune(e->s[m], reps, NULL, NULL, NULL)) return 1;
+    if(kb) kb[m] = e->s[m]->tune[TC_TUNE_PREDICT][0];
+    if(ty) ty[m] = e->s[m]->tune[TC_TUNE_PREDICT][1];
+  }
+  /* recorded graphs hold the old shapes */
+  if(e->recorded){
+    HIPOK(hipStreamSynchronize(e->stream));
+    drop_graphs(e);
+  }
+  return 0;
+}
+
+/* one eager step (so that every member's CFL state is the steady one), then the substeps of
+ * two steps recorded; nothing runs while recording, and the members' host state returns to
+ * where it was after the two recorded steps (even parity) */
+static int record(tc_ensemble_t * e){
+  for(int m = 0; m < e->n; m++){
+    if(tc_enqueue_step(e->s[m])) return 1;
+  }
+  HIPOK(hipStreamSynchronize(e->stream));
+  for(int p = 0; p < 2; p++){
+    for(int r = 0; r < 3; r++){
+      HIPOK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for(int m = 0; m < e->n && !rc; m++) rc = tc_enqueue_substep(e->s[m], r);
+      hipGraph_t g = NULL;
+      const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if(rc || ce != hipSuccess){
+        if(g) hipGraphDestroy(g);
+        drop_graphs(e);
+        return rc ? 1 : tc_fail("stream capture failed: %s", hipGetErrorString(ce));
+      }
+      e->graph[p][r] = g;
+      HIPOK(hipGraphInstantiate(&e->exec[p][r], g, NULL, NULL, 0));
+    }
+  }
+  e->recorded = 1;
+  e->steps = 0;
+  return 0;
+}
+
+int tc_ensemble_step(tc_ensemble_t * e, int nsteps){
+  HIPOK(hipSetDevice(e->device));
+  if(nsteps < 1) return 0;
+  int left = nsteps;
+  if(!e->recorded){
+    for(int m = 0; m < e->n; m++){
+      if(e->s[m]->prof_on || e->s[m]->diag_armed || e->s[m]->stat_armed)
+        return tc_fail("tc_ensemble_step: members may not profile or record diagnostics");
+    }
+    if(record(e)) return 1;   /* includes one eager step */
+    left -= 1;
+  }
+  for(int q = 0; q < left; q++){
+    const int p = (int)(e->steps & 1);
+    for(int r = 0; r < 3; r++) HIPOK(hipGraphLaunch(e->exec[p][r], e->stream));
+    for(int m = 0; m < e->n; m++) tc_advance_host_state(e->s[m]);
+    e->steps += 1;
+  }
+  return 0;
+}
+
+int tc_ensemble_synchronize(tc_ensemble_t * e){
+  HIPOK(hipSetDevice(e->device));
+  HIPOK(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+/* a member's fields were replaced from the host: its recorded step is still valid (same
+ * buffers), but the parity of the replay must match the member's buffers again */
+int tc_ensemble_invalidate(tc_ensemble_t * e){
+  HIPOK(hipSetDevice(e->device));
+  HIPOK(hipStreamSynchronize(e->stream));
+  drop_graphs(e);
+  return 0;
+}

@@ -40,6 +40,7 @@ struct tc_solver_s {
   tc_coef_t coef;
   tc_flags_t flags;
   hipStream_t stream;
+  int own_stream;         /* 0: the stream belongs to an ensemble */
   int tp;                 /* pitch of per-i tables: nx + 2 */
   int nky;
   double hz, nu, kappa;
@@ -84,12 +85,32 @@ struct tc_solver_s {
   int prof_on;
   tc_prof_t prof[TC_NSTAGE];
   int tune[TC_NTUNE][2];
+  /* learned closure (tc_cnn.c): network, corrections in the field layout (float), inputs */
+  tc_cnn_t * closure;
+  void * d_corr[TC_NFIELD];
+  double * d_cnn_xc;
+  float cnn_mean[5], cnn_inv_std[5], cnn_scale[4];
 };
 
 int tc_fail(const char * fmt, ...);
+/* return 1 with a message from the enclosing function when a HIP call or launcher fails */
+#define HIPOK(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess) \
+  return tc_fail("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); } while(0)
+#define KOK(expr) do { int e_ = (expr); if(e_ != 0) \
+  return tc_fail("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString((hipError_t)e_)); } while(0)
 int tc_apply_bc(tc_solver_t * s, int field, double * a);
 double * tc_field_ptr(tc_solver_t * s, int f);
 int tc_enqueue_step(tc_solver_t * s);
+/* substep r of a step (r = 0 also decides dt and the implicit factors) */
+int tc_enqueue_substep(tc_solver_t * s, int r);
+/* the host-side bookkeeping of one step (buffer parity, CFL state) without launching: for
+ * replays of a recorded step */
+void tc_advance_host_state(tc_solver_t * s);
+/* make the solver enqueue on a stream it does not own */
+int tc_share_stream(tc_solver_t * s, hipStream_t stream);
+/* the attached network on the current fields -> corrections (tc_cnn.c) */
+int tc_closure_eval(tc_solver_t * s);
+
 int tc_finish_field(tc_solver_t * s, int f);
 
 void tc_prof_begin(tc_solver_t * s, int stage);

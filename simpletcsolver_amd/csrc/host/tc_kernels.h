@@ -17,8 +17,8 @@
  * are evaluated in FP32; otherwise they are double and dlt may be nxt itself */
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     const double * const cur[4], const double * p, double * const nxt[4], void * const dlt[4],
-    const void * const srcb[4], void * const srca[4], const double * dt, int rkstep,
-    int final_here, int mixed, int kb, int ty, hipStream_t stream);
+    const void * const srcb[4], void * const srca[4], const float * const corr[4],
+    const double * dt, int rkstep, int final_here, int mixed, int kb, int ty, hipStream_t stream);
 int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     double * ux, double * uy, double * uz, double * p, const void * psi, const double * dt,
     int rkstep, int mode, unsigned long long * dtmin, int mixed, int kb, int ty, hipStream_t stream);

@@ -44,10 +44,6 @@ int tc_fail(const char * fmt, ...){
   return 1;
 }
 
-#define HIPOK(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess) \
-  return tc_fail("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); } while(0)
-#define KOK(expr) do { int e_ = (expr); if(e_ != 0) \
-  return tc_fail("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString((hipError_t)e_)); } while(0)
 
 void tc_params_default(tc_params_t * p){
   memset(p, 0, sizeof(*p));
@@ -366,6 +362,7 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
     free(s);
     return tc_fail("hipStreamCreate failed");
   }
+  s->own_stream = 1;
   tc_geom_t * g = &s->g;
   g->nx = p->nx; g->ny = p->ny; g->nz = p->nz;
   g->px = ((p->nx + 2 + TC_XOFF) + 15) / 16 * 16;
@@ -429,7 +426,9 @@ int tc_destroy(tc_solver_t * s){
       if(s->prof[n].ev[e][0]){ hipEventDestroy(s->prof[n].ev[e][0]); hipEventDestroy(s->prof[n].ev[e][1]); }
     }
   }
-  if(s->stream) hipStreamDestroy(s->stream);
+  for(int f = 0; f < TC_NFIELD; f++) if(s->d_corr[f]) hipFree(s->d_corr[f]);
+  if(s->d_cnn_xc) hipFree(s->d_cnn_xc);
+  if(s->stream && s->own_stream) hipStreamDestroy(s->stream);
   free(s->h_tab); free(s->h_xf); free(s->h_xc); free(s->h_pl); free(s->h_pu);
   free(s);
   return 0;
@@ -763,8 +762,10 @@ static int predict(tc_solver_t * s, int rkstep){
     srcb[f] = rkstep > 0 ? s->src[f][(rkstep - 1) & 1] : NULL;
     srca[f] = rkstep < 2 ? s->src[f][rkstep & 1] : NULL;
   }
+  const float * corr[4] = {NULL, NULL, NULL, NULL};
+  if(s->closure) for(int f = 0; f < 4; f++) corr[f] = (const float *)s->d_corr[f];
   PROF(TC_S_PREDICT, tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, dlt, srcb, srca,
-      s->d_dt, rkstep, !any_implicit, s->mixed, s->tune[TC_TUNE_PREDICT][0], s->tune[TC_TUNE_PREDICT][1], s->stream));
+      corr, s->d_dt, rkstep, !any_implicit, s->mixed, s->tune[TC_TUNE_PREDICT][0], s->tune[TC_TUNE_PREDICT][1], s->stream));
   if(any_implicit){
     int last = 0;
     for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
@@ -813,18 +814,98 @@ static int correct(tc_solver_t * s, int rkstep, int mode, int want_cfl){
   return 0;
 }
 
-int tc_enqueue_step(tc_solver_t * s){
-  if(launch_dt(s)) return 1;
-  if(factor_tables(s, 0, 2)) return 1;
-  for(int r = 0; r < 3; r++){
-    if(predict(s, r)) return 1;
-    if(potential(s, r)) return 1;
-    /* the last correction also yields the CFL minimum of the next step (decide_dt.c:32-80
-     * reads exactly the velocity this kernel writes) */
-    const int cfl = (r == 2) && !(s->p.dt_fixed > 0.);
-    if(correct(s, r, 3, cfl)) return 1;
-    if(cfl) s->dt_valid = 1;
+int tc_enqueue_substep(tc_solver_t * s, int r){
+  if(r == 0){
+    if(launch_dt(s)) return 1;
+    if(factor_tables(s, 0, 2)) return 1;
+    /* the learned correction of this step, from the fields at its start */
+    if(s->closure && tc_closure_eval(s)) return 1;
   }
+  if(predict(s, r)) return 1;
+  if(potential(s, r)) return 1;
+  /* the last correction also yields the CFL minimum of the next step (decide_dt.c:32-80
+   * reads exactly the velocity this kernel writes) */
+  const int cfl = (r == 2) && !(s->p.dt_fixed > 0.);
+  if(correct(s, r, 3, cfl)) return 1;
+  if(cfl) s->dt_valid = 1;
+  return 0;
+}
+
+int tc_enqueue_step(tc_solver_t * s){
+  for(int r = 0; r < 3; r++){
+    if(tc_enqueue_substep(s, r)) return 1;
+  }
+  return 0;
+}
+
+void tc_advance_host_state(tc_solver_t * s){
+  const int nf = s->p.scalar ? 4 : 3;
+  for(int r = 0; r < 3; r++){
+    for(int f = 0; f < nf; f++) s->cur[f] = 1 - s->cur[f];
+  }
+  s->dt_valid = !(s->p.dt_fixed > 0.);
+  s->diag_valid = 0;
+}
+
+int tc_share_stream(tc_solver_t * s, hipStream_t stream){
+  HIPOK(hipSetDevice(s->p.device));
+  HIPOK(hipStreamSynchronize(s->stream));
+  if(s->own_stream) HIPOK(hipStreamDestroy(s->stream));
+  s->stream = stream;
+  s->own_stream = 0;
+  return 0;
+}
+
+int tc_autotune(tc_solver_t * s, int reps, int * kb_out, int * ty_out, double * best_ms){
+  static const int kbs[5] = {2, 4, 8, 16, 32};
+  static const int tys[3] = {1, 2, 4};
+  HIPOK(hipSetDevice(s->p.device));
+  const int any_implicit = s->p.implicit[0] || s->p.implicit[1] || s->p.implicit[2];
+  const double * cur[4];
+  double * nxt[4];
+  void * dlt[4];
+  const void * srcb[4];
+  void * srca[4];
+  for(int f = 0; f < 4; f++){
+    cur[f] = s->u[f][s->cur[f]];
+    nxt[f] = s->u[f][1 - s->cur[f]];
+    dlt[f] = s->mixed ? s->dlt[f] : (void *)nxt[f];
+    srcb[f] = s->src[f][0];
+    srca[f] = s->src[f][1];
+  }
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  double best = 1e300;
+  int bkb = s->tune[TC_TUNE_PREDICT][0], bty = s->tune[TC_TUNE_PREDICT][1];
+  int rc = 0;
+  for(int a = 0; a < 5 && !rc; a++){
+    if(kbs[a] > s->p.nz) continue;
+    for(int b = 0; b < 3 && !rc; b++){
+      /* one untimed launch, then reps timed ones */
+      for(int q = 0; q <= reps && !rc; q++){
+        if(q == 1 && hipEventRecord(e0, s->stream) != hipSuccess) rc = 1;
+        if(!rc) rc = tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, dlt, srcb, srca,
+            NULL, s->d_dt, 1, !any_implicit, s->mixed, kbs[a], tys[b], s->stream);
+      }
+      float ms = 0.f;
+      if(!rc && (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) rc = 1;
+      if(!rc && ms / reps < best){
+        best = ms / reps;
+        bkb = kbs[a];
+        bty = tys[b];
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if(rc) return tc_fail("tc_autotune: launch or timing failed");
+  s->tune[TC_TUNE_PREDICT][0] = bkb; s->tune[TC_TUNE_PREDICT][1] = bty;
+  s->tune[TC_TUNE_CORRECT][0] = bkb; s->tune[TC_TUNE_CORRECT][1] = bty;
+  if(kb_out) *kb_out = bkb;
+  if(ty_out) *ty_out = bty;
+  if(best_ms) *best_ms = best;
   return 0;
 }
 

@@ -431,9 +431,23 @@ int tck_fft_batch(int M, int cap){
   return fft_tile(M, cap, sizeof(double2), &t);
 }
 
+/* raises a kernel's dynamic LDS limit once per (kernel, size): later launches, including
+ * those recorded into a graph, make no attribute call */
+#define FFT_LDS_CACHE 32
+static const void * g_lds_fn[FFT_LDS_CACHE];
+static size_t g_lds_bytes[FFT_LDS_CACHE];
+static int g_lds_n = 0;
+
 static int set_lds(const void * fn, size_t bytes){
   if(bytes <= 65536) return 0;
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess ? 0 : -1;
+  for(int q = 0; q < g_lds_n; q++) if(g_lds_fn[q] == fn && g_lds_bytes[q] >= bytes) return 0;
+  if(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return -1;
+  if(g_lds_n < FFT_LDS_CACHE){
+    g_lds_fn[g_lds_n] = fn;
+    g_lds_bytes[g_lds_n] = bytes;
+    g_lds_n++;
+  }
+  return 0;
 }
 
 /* mixed: spectrum, transforms and psi in FP32 (twiddles twM, twN, tw as float2) */

@@ -443,8 +443,9 @@ struct PSolveArgs {
 
 /* factor rows row0+e0 .. row0+e0+PS_PRE-1 of table t: rows below jc from the table (loaded
  * together, ahead of use), rows from jc on the constant cval, row m-1 lval */
-__device__ __forceinline__ void ps_rows(double * out, const double * __restrict__ t, long long ts,
-    int row0, int e0, int jc, int m, double cval, double lval){
+template <typename Q>
+__device__ __forceinline__ void ps_rows(Q * out, const double * __restrict__ t, long long ts,
+    int row0, int e0, int jc, int m, Q cval, Q lval){
   if(row0 + e0 < jc){
     double tmp[PS_PRE];
     #pragma unroll
@@ -455,7 +456,7 @@ __device__ __forceinline__ void ps_rows(double * out, const double * __restrict_
     #pragma unroll
     for(int e = 0; e < PS_PRE; e++){
       const int row = row0 + e0 + e;
-      out[e] = row < jc ? tmp[e] : (row == m - 1 ? lval : cval);
+      out[e] = row < jc ? Q(tmp[e]) : (row == m - 1 ? lval : cval);
     }
   }else{
     #pragma unroll
@@ -465,7 +466,7 @@ __device__ __forceinline__ void ps_rows(double * out, const double * __restrict_
 
 /* body with restrict-qualified pointers: loads of the old field and the tables may be
  * scheduled ahead of the stores to the increment array */
-template <typename D>
+template <typename D, int MAXM>
 __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict__ d, double * uout,
     const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
     const double * __restrict__ q1t, const double * __restrict__ meta, double * ps_sm){
@@ -492,34 +493,35 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   const double * r = rt + col;
   const double * v = vt + col;
   const double * q1 = q1t + col;
-  const double den = meta[col];
-  const double tl = meta[ts + col];
-  const double tu = meta[2 * ts + col];
+  typedef D Q;   /* arithmetic type: the increment's own */
+  const Q den = Q(meta[col]);
+  const Q tl = Q(meta[ts + col]);
+  const Q tu = Q(meta[2 * ts + col]);
   const int jc = (int)meta[3 * ts + col];
-  const double rc = meta[4 * ts + col];
-  const double vc = meta[5 * ts + col];
-  const double rlast = meta[6 * ts + col];
-  double * smA = ps_sm;                  /* [S][TI] */
-  double * smB = ps_sm + S * TI;         /* [S][TI] */
-  double * smX = ps_sm + 2 * S * TI;     /* [3][TI]: q0[0], q0[n-2], q[n-1] */
-  double q[PS_MAXM];
+  const Q rc = Q(meta[4 * ts + col]);
+  const Q vc = Q(meta[5 * ts + col]);
+  const Q rlast = Q(meta[6 * ts + col]);
+  Q * smA = (Q *)ps_sm;                  /* [S][TI] */
+  Q * smB = smA + S * TI;         /* [S][TI] */
+  Q * smX = smA + 2 * S * TI;     /* [3][TI]: q0[0], q0[n-2], q[n-1] */
+  Q q[MAXM];
   #pragma unroll
-  for(int e = 0; e < PS_MAXM; e++) q[e] = (valid && e < cnt) ? double(d[base + step * (row0 + e)]) : 0.;
-  if(valid && sg == S - 1) smX[2 * TI + ti] = double(d[base + step * m]);
-  double t[PS_PRE];
+  for(int e = 0; e < MAXM; e++) q[e] = (valid && e < cnt) ? Q(d[base + step * (row0 + e)]) : Q(0.);
+  if(valid && sg == S - 1) smX[2 * TI + ti] = Q(d[base + step * m]);
+  Q t[PS_PRE];
   /* forward: q0 = q0 / c0, q_i = r_i (q_i - l q_{i-1}) */
-  double a = 1., b = 0.;
+  Q a = Q(1.), b = Q(0.);
   #pragma unroll
-  for(int e0 = 0; e0 < PS_MAXM; e0 += PS_PRE){
+  for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
     if(e0 < cnt){
       ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
       #pragma unroll
       for(int e = 0; e < PS_PRE; e++){
         if(e0 + e < cnt){
           const int row = row0 + e0 + e;
-          if(row == 0){ a = 0.; b = q[e0 + e] / t[e]; }
+          if(row == 0){ a = Q(0.); b = q[e0 + e] / t[e]; }
           else {
-            const double ai = - t[e] * tl;
+            const Q ai = - t[e] * tl;
             b = ai * b + t[e] * q[e0 + e];
             a = ai * a;
           }
@@ -530,10 +532,10 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   smA[sg * TI + ti] = a;
   smB[sg * TI + ti] = b;
   __syncthreads();
-  double in = 0.;
+  Q in = Q(0.);
   for(int s2 = 0; s2 < sg; s2++) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
   #pragma unroll
-  for(int e0 = 0; e0 < PS_MAXM; e0 += PS_PRE){
+  for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
     if(e0 < cnt){
       ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
       #pragma unroll
@@ -550,19 +552,19 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   }
   __syncthreads();
   /* backward: q_i -= v_i q_{i+1}, q0[n-2] kept */
-  a = 1.;
-  b = 0.;
+  a = Q(1.);
+  b = Q(0.);
   #pragma unroll
-  for(int e0 = PS_MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
+  for(int e0 = MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
     if(e0 < cnt){
-      ps_rows(t, v, ts, row0, e0, jc, m, vc, 0.);
+      ps_rows(t, v, ts, row0, e0, jc, m, vc, Q(0.));
       #pragma unroll
       for(int e = PS_PRE - 1; e >= 0; e--){
         if(e0 + e < cnt){
           const int row = row0 + e0 + e;
-          if(row == m - 1){ a = 0.; b = q[e0 + e]; }
+          if(row == m - 1){ a = Q(0.); b = q[e0 + e]; }
           else {
-            const double ai = - t[e];
+            const Q ai = - t[e];
             b = ai * b + q[e0 + e];
             a = ai * a;
           }
@@ -573,12 +575,12 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   smA[sg * TI + ti] = a;
   smB[sg * TI + ti] = b;
   __syncthreads();
-  in = 0.;
+  in = Q(0.);
   for(int s2 = S - 1; s2 > sg; s2--) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
   #pragma unroll
-  for(int e0 = PS_MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
+  for(int e0 = MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
     if(e0 < cnt){
-      ps_rows(t, v, ts, row0, e0, jc, m, vc, 0.);
+      ps_rows(t, v, ts, row0, e0, jc, m, vc, Q(0.));
       #pragma unroll
       for(int e = PS_PRE - 1; e >= 0; e--){
         if(e0 + e < cnt){
@@ -592,29 +594,30 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   if(cnt > 0 && row0 == 0) smX[ti] = q[0];
   __syncthreads();
   /* x_{n-1} (tdm.c:92-94) and x_i += x_{n-1} q1_i (:96-98) */
-  const double num = smX[2 * TI + ti] - tu * smX[ti] - tl * smX[TI + ti];
-  const double xl = fabs(den) < DBL_EPSILON ? 0. : num / den;
+  const Q num = smX[2 * TI + ti] - tu * smX[ti] - tl * smX[TI + ti];
+  const Q xl = fabs(double(den)) < DBL_EPSILON ? Q(0.) : num / den;
   if(!valid) return;
   /* q1 and the old field, PS_PRE rows at a time */
   #pragma unroll
-  for(int e0 = 0; e0 < PS_MAXM; e0 += PS_PRE){
+  for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
     if(e0 < cnt){
-      double qq[PS_PRE], uu[PS_PRE];
+      Q qq[PS_PRE];
+      double uu[PS_PRE];
       #pragma unroll
       for(int e = 0; e < PS_PRE; e++){
         const int row = row0 + e0 + e < m ? row0 + e0 + e : m - 1;
-        qq[e] = q1[ts * row];
+        qq[e] = Q(q1[ts * row]);
         uu[e] = ucur ? ucur[base + step * row] : 0.;
       }
       #pragma unroll
       for(int e = 0; e < PS_PRE; e++){
         if(e0 + e < cnt){
           const int row = row0 + e0 + e;
-          const double x = q[e0 + e] + xl * qq[e];
+          const Q x = q[e0 + e] + xl * qq[e];
           if(ucur){
             const int jj = A.dir == 1 ? 1 + row : other;
             const int kk = A.dir == 1 ? other : 1 + row;
-            tck_store_halo(uout, g, i, jj, kk, uu[e] + x, false);
+            tck_store_halo(uout, g, i, jj, kk, uu[e] + double(x), false);
           }else{
             d[base + step * row] = D(x);
           }
@@ -626,17 +629,17 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
     if(ucur){
       const int jj = A.dir == 1 ? n : other;
       const int kk = A.dir == 1 ? other : n;
-      tck_store_halo(uout, g, i, jj, kk, ucur[base + step * m] + xl, false);
+      tck_store_halo(uout, g, i, jj, kk, ucur[base + step * m] + double(xl), false);
     }else{
       d[base + step * m] = D(xl);
     }
   }
 }
 
-template <typename D>
+template <typename D, int MAXM>
 __global__ void __launch_bounds__(512) tck_psolve(const PSolveArgs A){
   extern __shared__ double ps_sm[];
-  psolve_body<D>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.q1, A.meta, ps_sm);
+  psolve_body<D, MAXM>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.q1, A.meta, ps_sm);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -849,15 +852,20 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, void * d, double * uout, con
   A.tstride = tstride; A.tcol = tcol; A.dir = dir; A.i0 = i0; A.isz = isz;
   const int n = dir == 1 ? g->ny : g->nz;
   const int m = n - 1;
-  const int S = (m + PS_MAXM - 1) / PS_MAXM;
+  /* the mixed step keeps twice the rows per thread (FP32 registers) so that a row access of
+   * TI lanes is a whole 128 B line */
+  const int maxm = mixed ? 2 * PS_MAXM : PS_MAXM;
+  const int S = (m + maxm - 1) / maxm;
   if(S > 32) return (int)hipErrorInvalidValue;     /* n <= 1025: blocks of at most 512 threads */
+  const int timin = mixed ? 32 : 16;
   int TI = 512 / S;
-  TI = TI > 64 ? 64 : (TI < 16 ? 16 : TI);
+  TI = TI > 64 ? 64 : (TI < timin ? timin : TI);
+  if(TI * S > 512) return (int)hipErrorInvalidValue;
   const int nother = dir == 1 ? g->nz : g->ny;
   const size_t lds = sizeof(double) * (size_t)(2 * S * TI + 3 * TI);
   const int span = i0 + isz - 1;   /* tiles cover i = 1 .. i0+isz-1 */
-  if(mixed) hipLaunchKernelGGL(tck_psolve<float>, dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
-  else hipLaunchKernelGGL(tck_psolve<double>, dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
+  if(mixed) hipLaunchKernelGGL((tck_psolve<float, 2 * PS_MAXM>), dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
+  else hipLaunchKernelGGL((tck_psolve<double, PS_MAXM>), dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }

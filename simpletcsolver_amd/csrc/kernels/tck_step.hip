@@ -62,6 +62,8 @@ struct PredictArgs {
   void * dx; void * dy; void * dz; void * dtt;
   const void * sbx; const void * sby; const void * sbz; const void * sbt;
   void * sax; void * say; void * saz; void * sat;
+  /* learned corrections of the explicit right-hand sides (field layout, float), or null */
+  const float * cx; const float * cy; const float * cz; const float * ct;
   const double * dt;
   int rkstep;
   int final_here;   /* 1: no line solve follows, write u + delta with halos */
@@ -173,6 +175,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     }
     /* pressure :377-394 */
     gg -= CT(T_IHXF, i) * (- R(p[o - 1]) + R(p[o]));
+    if(A.cx) a += R(A.cx[kb0 + o]);
     R b = R(0.);
     if(sbx) b = sbx[o];
     const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -260,6 +263,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     }
     /* pressure :402-419 */
     gg -= CT(T_IHYC, i) * (- R(p[o - sy]) + R(p[o]));
+    if(A.cy) a += R(A.cy[kb0 + o]);
     R b = R(0.);
     if(sby) b = sby[o];
     const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -327,6 +331,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     }
     /* pressure :301-318 */
     gg -= R(c.ihz) * (- R(p[o - sz]) + R(p[o]));
+    if(A.cz) a += R(A.cz[kb0 + o]);
     R b = R(0.);
     if(sbz) b = sbz[o];
     const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -375,6 +380,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       const R vz = kappa * (+ R(c.lapz_l) * tzm + R(c.lapz_c) * t0 + R(c.lapz_u) * tzp);
       if(iz) gg += vz; else a += vz;
     }
+    if(A.ct) a += R(A.ct[kb0 + o]);
     R b = R(0.);
     if(sbt) b = sbt[o];
     const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -607,8 +613,8 @@ static int check_shape(int kb, int ty){
 
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     const double * const cur[4], const double * p, double * const nxt[4], void * const dlt[4],
-    const void * const srcb[4], void * const srca[4], const double * dt, int rkstep,
-    int final_here, int mixed, int kb, int ty, hipStream_t stream){
+    const void * const srcb[4], void * const srca[4], const float * const corr[4],
+    const double * dt, int rkstep, int final_here, int mixed, int kb, int ty, hipStream_t stream){
   if(check_shape(kb, ty)) return (int)hipErrorInvalidValue;
   PredictArgs A;
   A.g = *g; A.c = *c; A.f = *f;
@@ -617,6 +623,8 @@ int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
   A.dx = dlt[0]; A.dy = dlt[1]; A.dz = dlt[2]; A.dtt = dlt[3];
   A.sbx = srcb[0]; A.sby = srcb[1]; A.sbz = srcb[2]; A.sbt = srcb[3];
   A.sax = srca[0]; A.say = srca[1]; A.saz = srca[2]; A.sat = srca[3];
+  A.cx = corr ? corr[0] : nullptr; A.cy = corr ? corr[1] : nullptr;
+  A.cz = corr ? corr[2] : nullptr; A.ct = corr ? corr[3] : nullptr;
   A.dt = dt; A.rkstep = rkstep; A.final_here = final_here;
   A.tm = cell_tiles(*g, kb, ty);
   if(mixed) hipLaunchKernelGGL(tck_predict<float>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
