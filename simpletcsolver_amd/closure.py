@@ -132,7 +132,7 @@ class Pair:
         return g
 
 
-def generate(pair, n_samples, spacing, dt_coarse, states=None):
+def generate(pair, n_samples, spacing, dt_coarse, states=None, log=None):
     """n_samples (inputs, targets) pairs from the fine run, `spacing` coarse steps apart.
     The fine solver runs at dt_coarse / r; returns x [n][nz][ny][nx][5] and the target
     corrections y [n][nz][ny][nx][4] (FP32).  states: a list that receives the filtered
@@ -140,7 +140,9 @@ def generate(pair, n_samples, spacing, dt_coarse, states=None):
     r = pair.r
     pair.fine.set_dt_params(0.5, 0.5, dt_coarse / r)
     xs, ys = [], []
-    for _ in range(n_samples):
+    for m in range(n_samples):
+        if log and m % 8 == 0:
+            log(f"sample {m} of {n_samples}")
         f0 = box_filter(pair.fine_state(), r)
         for _ in range(r):
             pair.fine.step()

@@ -41,6 +41,8 @@ struct tc_solver_s {
   tc_flags_t flags;
   hipStream_t stream;
   int own_stream;         /* 0: the stream belongs to an ensemble */
+  void * h_stage;         /* pinned staging of field transfers (whole planes) */
+  size_t h_stage_bytes;
   int tp;                 /* pitch of per-i tables: nx + 2 */
   int nky;
   double hz, nu, kappa;

@@ -428,6 +428,7 @@ int tc_destroy(tc_solver_t * s){
   }
   for(int f = 0; f < TC_NFIELD; f++) if(s->d_corr[f]) hipFree(s->d_corr[f]);
   if(s->d_cnn_xc) hipFree(s->d_cnn_xc);
+  if(s->h_stage) hipHostFree(s->h_stage);
   if(s->stream && s->own_stream) hipStreamDestroy(s->stream);
   free(s->h_tab); free(s->h_xf); free(s->h_xc); free(s->h_pl); free(s->h_pu);
   free(s);
@@ -478,6 +479,66 @@ static void npy_x(const tc_solver_t * s, int f, int * nxs, int * i0){
   else { *nxs = s->p.nx + 2; *i0 = 0; }
 }
 
+/* ---------------------------------------------------------------------------------------
+ * host transfers of fields: whole planes through a pinned staging buffer (at most
+ * TC_STAGE_BYTES, at least one plane), rows repacked on the host.  One contiguous copy per
+ * chunk of planes instead of one strided copy per row keeps PCIe busy at any grid size.
+ * ------------------------------------------------------------------------------------- */
+#define TC_STAGE_BYTES ((size_t)64 << 20)
+
+static int stage_buffer(tc_solver_t * s, size_t plane_bytes, size_t total, void ** out){
+  size_t want = total < TC_STAGE_BYTES ? total : TC_STAGE_BYTES;
+  if(want < plane_bytes) want = plane_bytes;
+  if(s->h_stage_bytes < want){
+    if(s->h_stage) hipHostFree(s->h_stage);
+    s->h_stage = NULL;
+    s->h_stage_bytes = 0;
+    HIPOK(hipHostMalloc(&s->h_stage, want, hipHostMallocDefault));
+    s->h_stage_bytes = want;
+  }
+  *out = s->h_stage;
+  return 0;
+}
+
+/* rows j0..j1 of planes k0..k1 of a device field (elements of esz bytes, nxs values from
+ * slot i0) <-> host rows at host_row(j, k) * nxs.  to_host: device -> host (widened from
+ * FP32 to FP64 when widen); otherwise host -> device, the planes read first so that what
+ * lies between the rows (halos, padding) is kept. */
+typedef size_t (*host_row_fn)(const tc_geom_t * g, int j, int k);
+static size_t row_npy(const tc_geom_t * g, int j, int k){ return (size_t)(k - 1) * g->ny + (size_t)(j - 1); }
+static size_t row_ref(const tc_geom_t * g, int j, int k){ return (size_t)k * (g->ny + 2) + (size_t)j; }
+
+static int xfer_rows(tc_solver_t * s, void * dev, size_t esz, int nxs, int i0, int j0, int j1,
+    int k0, int k1, host_row_fn host_row, void * host, int to_host, int widen){
+  const tc_geom_t * g = &s->g;
+  const size_t plane = (size_t)g->px * g->py;
+  const size_t plane_bytes = plane * esz;
+  void * stage = NULL;
+  if(stage_buffer(s, plane_bytes, plane_bytes * (size_t)(k1 - k0 + 1), &stage)) return 1;
+  const int nk = (int)(s->h_stage_bytes / plane_bytes);
+  const size_t hesz = widen ? sizeof(double) : esz;
+  for(int kc = k0; kc <= k1; kc += nk){
+    const int nc = kc + nk - 1 <= k1 ? nk : k1 - kc + 1;
+    char * dp = (char *)dev + (size_t)kc * plane_bytes;
+    HIPOK(hipMemcpyAsync(stage, dp, (size_t)nc * plane_bytes, hipMemcpyDeviceToHost, s->stream));
+    HIPOK(hipStreamSynchronize(s->stream));
+    for(int k = kc; k < kc + nc; k++){
+      for(int j = j0; j <= j1; j++){
+        char * sr = (char *)stage + ((size_t)(k - kc) * plane + (size_t)j * g->px + (size_t)(i0 + TC_XOFF)) * esz;
+        char * hr = (char *)host + host_row(g, j, k) * (size_t)nxs * hesz;
+        if(!to_host) memcpy(sr, hr, (size_t)nxs * esz);
+        else if(!widen) memcpy(hr, sr, (size_t)nxs * esz);
+        else for(int q = 0; q < nxs; q++) ((double *)hr)[q] = (double)((const float *)sr)[q];
+      }
+    }
+    if(!to_host){
+      HIPOK(hipMemcpyAsync(dp, stage, (size_t)nc * plane_bytes, hipMemcpyHostToDevice, s->stream));
+      HIPOK(hipStreamSynchronize(s->stream));
+    }
+  }
+  return 0;
+}
+
 /* layout 0: NPY (nz, ny, nxs); layout 1: the reference's host array with one halo plane on
  * each side in y and z, (nz+2, ny+2, nxs) (include/array_macros/fluid/*.h) */
 static int set_field_layout(tc_solver_t * s, const char * name, const double * data, int layout){
@@ -488,14 +549,8 @@ static int set_field_layout(tc_solver_t * s, const char * name, const double * d
   int nxs, i0;
   npy_x(s, f, &nxs, &i0);
   const tc_geom_t * g = &s->g;
-  double * dst = tc_field_ptr(s, f);
-  for(int k = 1; k <= g->nz; k++){
-    const double * src = layout == 0 ? data + (size_t)(k - 1) * g->ny * nxs
-                                     : data + (size_t)nxs * (1 + (size_t)(g->ny + 2) * k);
-    double * d = dst + (size_t)(i0 + TC_XOFF) + (size_t)g->px * (1 + (size_t)g->py * k);
-    HIPOK(hipMemcpy2DAsync(d, sizeof(double) * g->px, src, sizeof(double) * nxs,
-        sizeof(double) * nxs, g->ny, hipMemcpyHostToDevice, s->stream));
-  }
+  if(xfer_rows(s, tc_field_ptr(s, f), sizeof(double), nxs, i0, 1, g->ny, 1, g->nz,
+      layout == 0 ? row_npy : row_ref, (void *)data, 0, 0)) return 1;
   return tc_finish_field(s, f);
 }
 
@@ -522,57 +577,19 @@ int tc_finish_field(tc_solver_t * s, int f){
   return 0;
 }
 
-/* psi of the mixed step: FP32 rows, widened on the host */
-static int get_psi32(tc_solver_t * s, double * data, int layout){
-  const tc_geom_t * g = &s->g;
-  int nxs, i0;
-  npy_x(s, TC_FPSI, &nxs, &i0);
-  const size_t rows = layout == 0 ? (size_t)g->ny * g->nz : (size_t)g->py * g->pz;
-  float * tmp = malloc(sizeof(float) * rows * nxs);
-  if(!tmp) return tc_fail("host allocation failed");
-  const float * src = s->d_psi32 + i0 + TC_XOFF;
-  hipError_t e = hipSuccess;
-  if(layout == 0){
-    for(int k = 1; k <= g->nz && e == hipSuccess; k++){
-      e = hipMemcpy2DAsync(tmp + (size_t)(k - 1) * g->ny * nxs, sizeof(float) * nxs,
-          src + (size_t)g->px * (1 + (size_t)g->py * k), sizeof(float) * g->px,
-          sizeof(float) * nxs, g->ny, hipMemcpyDeviceToHost, s->stream);
-    }
-  }else{
-    e = hipMemcpy2DAsync(tmp, sizeof(float) * nxs, src, sizeof(float) * g->px, sizeof(float) * nxs,
-        rows, hipMemcpyDeviceToHost, s->stream);
-  }
-  if(e == hipSuccess) e = hipStreamSynchronize(s->stream);
-  if(e == hipSuccess) for(size_t q = 0; q < rows * nxs; q++) data[q] = (double)tmp[q];
-  free(tmp);
-  if(e != hipSuccess) return tc_fail("psi download: %s", hipGetErrorString(e));
-  return 0;
-}
-
 static int get_field_layout(tc_solver_t * s, const char * name, double * data, int layout){
   const int f = field_id(name);
   if(f < 0) return tc_fail("unknown field '%s'", name);
   HIPOK(hipSetDevice(s->p.device));
-  if(f == TC_FPSI && s->mixed) return get_psi32(s, data, layout);
   int nxs, i0;
   npy_x(s, f, &nxs, &i0);
   const tc_geom_t * g = &s->g;
-  const double * src = tc_field_ptr(s, f);
-  if(layout == 0){
-    for(int k = 1; k <= g->nz; k++){
-      double * d = data + (size_t)(k - 1) * g->ny * nxs;
-      const double * sp = src + (size_t)(i0 + TC_XOFF) + (size_t)g->px * (1 + (size_t)g->py * k);
-      HIPOK(hipMemcpy2DAsync(d, sizeof(double) * nxs, sp, sizeof(double) * g->px,
-          sizeof(double) * nxs, g->ny, hipMemcpyDeviceToHost, s->stream));
-    }
-  }else{
-    /* every row of every plane, halos included */
-    const double * sp = src + (size_t)(i0 + TC_XOFF);
-    HIPOK(hipMemcpy2DAsync(data, sizeof(double) * nxs, sp, sizeof(double) * g->px,
-        sizeof(double) * nxs, (size_t)g->py * g->pz, hipMemcpyDeviceToHost, s->stream));
-  }
-  HIPOK(hipStreamSynchronize(s->stream));
-  return 0;
+  /* psi of the mixed step is FP32, widened on the host */
+  const int w32 = f == TC_FPSI && s->mixed;
+  void * src = w32 ? (void *)s->d_psi32 : (void *)tc_field_ptr(s, f);
+  const size_t esz = w32 ? sizeof(float) : sizeof(double);
+  if(layout == 0) return xfer_rows(s, src, esz, nxs, i0, 1, g->ny, 1, g->nz, row_npy, data, 1, w32);
+  return xfer_rows(s, src, esz, nxs, i0, 0, g->py - 1, 0, g->pz - 1, row_ref, data, 1, w32);
 }
 
 int tc_get_field(tc_solver_t * s, const char * name, double * data){

@@ -35,6 +35,32 @@ def test_closure_reduces_the_coarse_step_error(need_gpu):
     assert again == base                      # detaching restores the plain method exactly
     assert losses[-1] < 0.5 * losses[0]
     assert with_closure < 0.9 * base
+    _graph_replay_matches_eager(pair, states[-1][0], net, norm, dt, lx, re)
     one.close()
     net.close()
     pair.close()
+
+
+def _graph_replay_matches_eager(pair, f0, net, norm, dt, lx, re):
+    """the closure's kernels recorded into the step graphs give the eager steps bit for bit"""
+    from simpletcsolver_amd import Ensemble, closure
+    spec = dict(nx=pair.nxc, ny=pair.nyc, nz=pair.nzc, xf=pair.xfc, xc=pair.xcc, is_curved=True,
+                lengths=(lx, np.pi / 3.0, 2.0 * lx), Re=re, implicit=(1, 0, 0), scalar=1, dt_fixed=dt)
+    ens = Ensemble([spec])
+    m = ens.members[0]
+    m.load_fields(f0)
+    one = closure.attach(m, net, norm)
+    pair.coarse.set_dt_params(0.5, 0.5, dt)
+    pair.coarse.load_fields(f0)
+    ref = closure.attach(pair.coarse, net, norm)
+    for step in range(4):
+        ens.step(1)
+        ens.synchronize()
+        pair.coarse.step()
+        got, want = m.fields(), pair.coarse.fields()
+        for n in ("ux", "uy", "uz", "t", "p"):
+            assert np.array_equal(got[n], want[n]), (step, n)
+    closure.detach(pair.coarse)
+    ens.close()
+    one.close()
+    ref.close()
