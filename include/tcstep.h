@@ -29,6 +29,10 @@ extern "C" {
  * and diagnostics; FP32 right-hand sides, RK sources, implicit-solve increments, FFTs and
  * potential */
 #define TC_MIXED 1
+/* no reference counterpart: TC_MIXED plus one defect-correction pass at the end of every
+ * step -- the divergence left by the FP32 potential is measured in FP64, solved for once more
+ * and removed from the velocity -- so that the step leaves FP64-level divergence */
+#define TC_MIXED_REFINED 2
 
 typedef struct {
   int nx, ny, nz;           /* glsizes (domain.c:40) */
@@ -42,7 +46,7 @@ typedef struct {
   double coef_dt_adv;       /* decide_dt.c:128 */
   double coef_dt_dif;       /* decide_dt.c:129 */
   double dt_fixed;          /* > 0: decide_dt returns this value (no reference counterpart) */
-  int precision;            /* TC_FP64 or TC_MIXED */
+  int precision;            /* TC_FP64, TC_MIXED or TC_MIXED_REFINED */
   int device;               /* HIP device ordinal */
 } tc_params_t;
 

@@ -36,7 +36,7 @@ EXPORTED = (
 CNN_SHAPES = ((27, 32, 16), (27, 32, 32), (27, 16, 32))
 
 # tc_params_t.precision (include/tcstep.h)
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed_refined": 2}
 
 # running statistics of statistics.c, in its accumulation order
 STATS = ("ux1", "ux2", "uy1", "uy2", "uz1", "uz2", "adv", "dif")

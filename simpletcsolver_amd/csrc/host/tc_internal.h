@@ -52,6 +52,8 @@ struct tc_solver_s {
   int cur[TC_NFIELD];
   int mixed;              /* precision TC_MIXED: FP32 temporaries (src, increment, spectrum,
                            * psi), FP64 state and projection arithmetic */
+  int refine;             /* TC_MIXED_REFINED: a second potential and velocity correction
+                           * at the last substep */
   size_t esz;             /* bytes per temporary element: 8 or 4 */
   void * src[TC_NFIELD][2];
   void * dlt[TC_NFIELD];  /* the increment of the mixed step (FP64: the next field buffer) */

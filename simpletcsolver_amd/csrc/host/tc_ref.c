@@ -347,6 +347,7 @@ int fluid_init(const char dirname_ic[], const domain_t * domain, fluid_t * fluid
   p.t_xm = param_t_xm; p.t_xp = param_t_xp;
   const char * prec = getenv("TC_PRECISION");
   if(prec && strcmp(prec, "mixed") == 0) p.precision = TC_MIXED;
+  if(prec && strcmp(prec, "mixed_refined") == 0) p.precision = TC_MIXED_REFINED;
   int slot = -1;
   for(int n = 0; n < TC_MAXREG; n++) if(!g_reg[n].s){ slot = n; break; }
   if(slot < 0){

@@ -346,13 +346,14 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
     return tc_fail("ny/2 = %d and nz = %d must be products of 2 and 3", p->ny / 2, p->nz);
   if(p->nx > 1024) return tc_fail("nx = %d exceeds 1024 (radial line solver)", p->nx);
   if(p->ny / 2 > 4096 || p->nz > 4096) return tc_fail("transform length exceeds 4096");
-  if(p->precision != TC_FP64 && p->precision != TC_MIXED) return tc_fail("precision %d not supported", p->precision);
+  if(p->precision < TC_FP64 || p->precision > TC_MIXED_REFINED) return tc_fail("precision %d not supported", p->precision);
   tc_solver_t * s = calloc(1, sizeof(tc_solver_t));
   if(!s) return tc_fail("host allocation failed");
   s->p = *p;
   s->tune[TC_TUNE_PREDICT][0] = 16; s->tune[TC_TUNE_PREDICT][1] = 4;
   s->tune[TC_TUNE_CORRECT][0] = 8; s->tune[TC_TUNE_CORRECT][1] = 4;
-  s->mixed = p->precision == TC_MIXED;
+  s->mixed = p->precision == TC_MIXED || p->precision == TC_MIXED_REFINED;
+  s->refine = p->precision == TC_MIXED_REFINED;
   s->esz = s->mixed ? sizeof(float) : sizeof(double);
   if(hipSetDevice(p->device) != hipSuccess){
     free(s);
@@ -843,7 +844,14 @@ int tc_enqueue_substep(tc_solver_t * s, int r){
   /* the last correction also yields the CFL minimum of the next step (decide_dt.c:32-80
    * reads exactly the velocity this kernel writes) */
   const int cfl = (r == 2) && !(s->p.dt_fixed > 0.);
-  if(correct(s, r, 3, cfl)) return 1;
+  const int refine = s->refine && r == 2;
+  if(correct(s, r, 3, cfl && !refine)) return 1;
+  if(refine){
+    /* defect correction: the FP64 divergence of the corrected velocity, its potential and a
+     * second velocity correction (the pressure keeps the first potential) */
+    if(potential(s, r)) return 1;
+    if(correct(s, r, 1, cfl)) return 1;
+  }
   if(cfl) s->dt_valid = 1;
   return 0;
 }

@@ -336,28 +336,35 @@ struct XSolveArgs {
   int i0, n;
 };
 
+/* LDS ordering between the lanes of one wavefront (the staging areas below are private to a
+ * wave, so the waves of a block need not wait for each other) */
+__device__ __forceinline__ void wave_sync(){
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* Four waves per block, one x line per wave per iteration; blocks loop over groups of four
- * lines, so the factor rows of each lane's segment (the same for every line) are loaded once
- * per block into registers.  Lines are staged through LDS: global loads and stores are
- * contiguous across the wave (lane t touches elements t, t+64, ...), each lane solves its
- * contiguous segment. */
+ * lines.  The factor rows (the same for every line) are staged once per block in LDS, in the
+ * lanes' segment order, and read where the recurrences use them: held in registers for the
+ * whole loop they would cost 3 SEG doubles per lane and occupancy.  Lines are staged through
+ * a wave-private LDS area: global loads and stores are contiguous across the wave (lane t
+ * touches elements t, t+64, ...), each lane solves its contiguous segment. */
 template <int SEG, typename D>
 __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict__ dd,
     double * uout, const double * __restrict__ ucur, const double * __restrict__ rt,
-    const double * __restrict__ vt, const double * __restrict__ lt, double * sm){
+    const double * __restrict__ vt, const double * __restrict__ lt, double * sm, double (*tab)[TCK_WAVE * (SEG + 1)]){
   const tc_geom_t & g = A.g;
   const int w = threadIdx.x / TCK_WAVE;
   const int lane = threadIdx.x % TCK_WAVE;
   const int n = A.n;
-  const int row0 = lane * SEG;
-  double rr[SEG], ll[SEG], vv[SEG];
-  #pragma unroll
-  for(int e = 0; e < SEG; e++){
-    const int row = row0 + e < n ? row0 + e : n - 1;
-    rr[e] = rt[row];
-    ll[e] = lt[row];
-    vv[e] = vt[row];
+  for(int idx = threadIdx.x; idx < TCK_WAVE * SEG; idx += blockDim.x){
+    const int row = idx < n ? idx : n - 1;
+    tab[0][stage_pos<SEG>(idx)] = rt[row];
+    tab[1][stage_pos<SEG>(idx)] = lt[row];
+    tab[2][stage_pos<SEG>(idx)] = vt[row];
   }
+  __syncthreads();
   const long long nlines = (long long)g.ny * g.nz;
   const int wpb = TCK_BLOCK / TCK_WAVE;
   for(long long g0 = (long long)blockIdx.x * wpb; g0 < nlines; g0 += (long long)gridDim.x * wpb){
@@ -381,14 +388,17 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
         }
       }
     }
-    __syncthreads();
+    wave_sync();
+    /* the table offset is opaque to the compiler, so the reads stay inside the loop */
+    int toff = lane * (SEG + 1);
+    asm volatile("" : "+v"(toff));
     double q[SEG][1];
     #pragma unroll
     for(int e = 0; e < SEG; e++) q[e][0] = sm[lane * (SEG + 1) + e];
-    wave_tdm<SEG, 1>(q, rr, ll, vv, n);
+    wave_tdm<SEG, 1>(q, &tab[0][toff], &tab[1][toff], &tab[2][toff], n);
     #pragma unroll
     for(int e = 0; e < SEG; e++) sm[lane * (SEG + 1) + e] = q[e][0];
-    __syncthreads();
+    wave_sync();
     if(live){
       if(ucur){
         #pragma unroll
@@ -404,14 +414,15 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
         }
       }
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
 template <int SEG, typename D>
 __global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
   __shared__ double xs_sm[TCK_BLOCK / TCK_WAVE][TCK_WAVE * (SEG + 1)];
-  xsolve_body<SEG, D>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE]);
+  __shared__ double xs_tab[3][TCK_WAVE * (SEG + 1)];
+  xsolve_body<SEG, D>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE], xs_tab);
 }
 
 /* periodic lines in y (solve_in_y, :39-70) or z (solve_in_z, :72-103).  A block holds TI

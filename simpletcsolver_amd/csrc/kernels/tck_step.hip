@@ -391,7 +391,11 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
 }
 
 template <typename R>
-__global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) tck_predict(const PredictArgs A){
+#ifndef TCK_PREDICT_WAVES
+#define TCK_PREDICT_WAVES 4   /* register budget of the predictor: waves per SIMD (4 with a
+                                 few spilled dwords measures faster than 3 without) */
+#endif
+__global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(TCK_PREDICT_WAVES))) tck_predict(const PredictArgs A){
   __shared__ R smt[T_NPRED * TCK_TW];
   int bx, by, bz;
   tile_of(A.tm, &bx, &by, &bz);

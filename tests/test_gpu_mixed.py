@@ -87,3 +87,29 @@ def test_mixed_torque_within_a_tenth_of_a_percent(need_gpu):
     assert rel <= 1e-3
     s64.close()
     s32.close()
+
+
+@pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 1, 1)])
+def test_mixed_refined_leaves_fp64_level_divergence(need_gpu, implicit):
+    """TC_MIXED_REFINED: one defect-correction pass at the end of every step takes the
+    divergence left by the FP32 potential down to the FP64 level (max |div u| <= 1e-10 after
+    every step), and the fields stay as close to the FP64 oracle as the plain mixed step's"""
+    case = make_case(nx=24, ny=16, nz=12, Re=500.0, implicit=implicit, amp=0.1, t_amp=0.05)
+    o = oracle_for(case)
+    s = _gpu(case, "mixed_refined")
+    plain = _gpu(case, "mixed")
+    worst = 0.0
+    for _ in range(10):
+        o.integrate()
+        s.step()
+        plain.step()
+        worst = max(worst, s.divergence())
+    of, sf = o.npy_fields(), s.fields()
+    errs = {n: rel_l2(sf[n], of[n]) for n in ("ux", "uy", "uz", "t")}
+    print(f"refined: max |div u| over 10 steps {worst:.2e} (plain mixed {plain.divergence():.2e}); "
+          "relative L2 vs FP64:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert worst <= 1e-10
+    for n in errs:
+        assert errs[n] <= 1e-5, n
+    s.close()
+    plain.close()
