@@ -9,6 +9,7 @@
 #include "../tc_shared.h"
 
 #define TCK_WAVE 64
+#define TCK_NXCD 8        /* XCDs: blocks are dispatched to them round-robin */
 
 /* launch-bounds hint used by every kernel below; keeps register allocation predictable */
 #define TCK_BLOCK 256

@@ -174,6 +174,14 @@ __device__ void lds_fft(typename cvec<T>::type * buf, int ld, int nb, const FftP
 __constant__ double c_fft_rka[3] = {32. / 60., 25. / 60., 45. / 60.};
 __constant__ double c_fft_rkb[3] = { 0. / 60., -17. / 60., -25. / 60.};
 
+/* tile of this block among ntiles: blocks that run on the same XCD (consecutive slots of the
+ * round-robin dispatch) take consecutive tiles, whose rows share 128 B lines in its L2 */
+__device__ __forceinline__ long long fft_tile_of(long long ntiles){
+  long long t = blockIdx.x;
+  if(ntiles % TCK_NXCD == 0) t = (t % TCK_NXCD) * (ntiles / TCK_NXCD) + t / TCK_NXCD;
+  return t;
+}
+
 /* ------------------------------------------------------------------------------------- */
 struct YFwdArgs {
   tc_geom_t g;
@@ -200,8 +208,10 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
   const tc_coef_t & c = A.c;
   const int ti = A.ti;
   const int M = A.P.M;
-  const int i0 = 1 + blockIdx.x * ti;
-  const int k = 1 + blockIdx.y;
+  const int ntx = (g.nx + ti - 1) / ti;
+  const long long tile = fft_tile_of((long long)ntx * g.nz);
+  const int i0 = 1 + (int)(tile % ntx) * ti;
+  const int k = 1 + (int)(tile / ntx);
   const double dt = *A.dt;
   const double gamma = c_fft_rka[A.rkstep] + c_fft_rkb[A.rkstep];
   const double norm = 1. * g.ny * g.nz;
@@ -256,6 +266,7 @@ struct ZArgs {
   double sign;
 };
 
+/* one block per tile of tz consecutive i at one ky */
 template <typename T>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   typedef typename cvec<T>::type V;
@@ -263,8 +274,10 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   V * sm = (V *)fft_smem;
   const int tz = A.tz;
   const int nz = A.nz;
-  const int i0 = blockIdx.x * tz;
-  const int ky = blockIdx.y;
+  const int ntx = (A.nx + tz - 1) / tz;
+  const long long tile = fft_tile_of((long long)ntx * A.nky);
+  const int i0 = (int)(tile % ntx) * tz;
+  const int ky = (int)(tile / ntx);
   const long long kstride = (long long)A.nx * A.nky;
   V * base = (V *)A.S + (long long)A.nx * ky;
   for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
@@ -316,8 +329,10 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
   const tc_geom_t & g = A.g;
   const int ti = A.ti;
   const int M = A.P.M;
-  const int i0 = 1 + blockIdx.x * ti;
-  const int k = 1 + blockIdx.y;
+  const int ntx = (g.nx + ti - 1) / ti;
+  const long long tile = fft_tile_of((long long)ntx * g.nz);
+  const int i0 = 1 + (int)(tile % ntx) * ti;
+  const int k = 1 + (int)(tile / ntx);
   const long long rowbase = (long long)g.nx * (long long)A.nky * (k - 1);
   const V * S = (const V *)A.S;
   const V * twN = (const V *)A.twN;
@@ -399,6 +414,12 @@ int tck_fft_supported(int n){
 /* Tile of TB transforms of length M per block: TB = 16 (one 128 B line of doubles, or two
  * lines of complex values, per row access) when the LDS allows it, fewer for very long
  * transforms; threads = TB * M / FFT_EPT. */
+#ifndef TCK_FFT_YSHARE
+#define TCK_FFT_YSHARE 1     /* theta transforms: blocks per CU the tiles are sized for */
+#endif
+#ifndef TCK_FFT_ZSHARE
+#define TCK_FFT_ZSHARE 1     /* axial transforms */
+#endif
 static size_t g_lds_limit = 0;
 
 static size_t lds_limit(void){
@@ -414,9 +435,12 @@ static size_t lds_limit(void){
   return g_lds_limit;
 }
 
-static int fft_tile(int M, int cap, size_t csize, int * threads){
+/* share: tiles sized to leave room for `share` blocks per CU in the LDS (while at least 4
+ * transforms, 64 B of contiguous FP64 complex values, remain per row access) */
+static int fft_tile(int M, int cap, size_t csize, int * threads, int share){
   int b = 16;
   while(b > 1 && ((size_t)b * (M + 1) * csize > lds_limit() || b * M > FFT_EPT * FFT_MAXTHREADS)) b /= 2;
+  while(share > 1 && b > 4 && (size_t)b * (M + 1) * csize * share > lds_limit()) b /= 2;
   while(b * 2 <= cap && b * 2 * M <= FFT_EPT * 64) b *= 2;   /* short transforms: wider tiles */
   if(b > cap) b = cap < 1 ? 1 : cap;
   int t = (b * M + FFT_EPT - 1) / FFT_EPT;
@@ -428,7 +452,7 @@ static int fft_tile(int M, int cap, size_t csize, int * threads){
 
 int tck_fft_batch(int M, int cap){
   int t;
-  return fft_tile(M, cap, sizeof(double2), &t);
+  return fft_tile(M, cap, sizeof(double2), &t, 1);
 }
 
 /* raises a kernel's dynamic LDS limit once per (kernel, size): later launches, including
@@ -460,11 +484,11 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.ti = fft_tile(A.P.M, 64, cs, &nt);
+  A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE);
   A.twM = twM; A.twN = twN;
   A.ld = A.P.M + 1;
   const size_t lds = cs * (size_t)A.ti * A.ld;
-  const dim3 grd((g->nx + A.ti - 1) / A.ti, g->nz);
+  const dim3 grd((unsigned)((long long)((g->nx + A.ti - 1) / A.ti) * g->nz));
   if(mixed){
     if(set_lds((const void *)tck_div_yfft<float>, lds)) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(tck_div_yfft<float>, grd, dim3(nt), lds, stream, A);
@@ -483,10 +507,10 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
   if(make_plan(nz, &A.P) != 0) return (int)hipErrorInvalidValue;
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.tz = fft_tile(nz, 64, cs, &nt);
+  A.tz = fft_tile(nz, 64, cs, &nt, TCK_FFT_ZSHARE);
   A.ld = nz + 1;
   const size_t lds = cs * (size_t)A.tz * A.ld;
-  const dim3 grd((nx + A.tz - 1) / A.tz, nky);
+  const dim3 grd((unsigned)((long long)((nx + A.tz - 1) / A.tz) * nky));
   if(mixed){
     if(set_lds((const void *)tck_zfft<float>, lds)) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(tck_zfft<float>, grd, dim3(nt), lds, stream, A);
@@ -505,11 +529,11 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.ti = fft_tile(A.P.M, 64, cs, &nt);
+  A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE);
   A.twM = twM; A.twN = twN;
   A.ld = A.P.M + 1;
   const size_t lds = cs * (size_t)A.ti * A.ld;
-  const dim3 grd((g->nx + A.ti - 1) / A.ti, g->nz);
+  const dim3 grd((unsigned)((long long)((g->nx + A.ti - 1) / A.ti) * g->nz));
   if(mixed){
     if(set_lds((const void *)tck_yifft_psi<float>, lds)) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(tck_yifft_psi<float>, grd, dim3(nt), lds, stream, A);

@@ -29,7 +29,6 @@ __constant__ double c_rkb[3] = { 0. / 60., -17. / 60., -25. / 60.};
  * chunk) and renumbered so that each of the 8 XCDs (which receive launch-order block ids
  * round-robin and keep their own L2) works through one contiguous eighth of the tiles: the
  * halo rows and planes a block shares with its neighbours then sit in the same L2. */
-#define TCK_NXCD 8
 #define TCK_DTSLOTS 64      /* CFL minimum accumulators, one 128 B line apart */
 #define TCK_DTSTRIDE 16
 
@@ -390,12 +389,11 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   }
 }
 
+/* register budget of the predictor, waves per SIMD: FP64 measures faster at 4 (a few dwords
+ * spilled) than at 3, the mixed step's FP32 instantiation slower */
+#define TCK_PREDICT_WAVES(R) (sizeof(R) == 8 ? 4 : 3)
 template <typename R>
-#ifndef TCK_PREDICT_WAVES
-#define TCK_PREDICT_WAVES 4   /* register budget of the predictor: waves per SIMD (4 with a
-                                 few spilled dwords measures faster than 3 without) */
-#endif
-__global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(TCK_PREDICT_WAVES))) tck_predict(const PredictArgs A){
+__global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(TCK_PREDICT_WAVES(R)))) tck_predict(const PredictArgs A){
   __shared__ R smt[T_NPRED * TCK_TW];
   int bx, by, bz;
   tile_of(A.tm, &bx, &by, &bz);
