@@ -338,6 +338,30 @@ static int setup_linesolve(tc_solver_t * s){
   return 0;
 }
 
+/* launch shapes of the cell kernels (planes per block kb, rows per block ty): 16 x 4 for the
+ * predictor and 8 x 4 for the correction, shortened until the grid has at least four blocks
+ * per CU, so that small grids fill the device (tc_autotune refines them) */
+static void default_shapes(tc_solver_t * s){
+  int ncu = 0;
+  if(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->p.device) != hipSuccess || ncu <= 0) ncu = 256;
+  const long long want = 4LL * ncu;
+  const long long nbx = (s->p.nx + 63) / 64;
+  const int start[2][2] = {{16, 4}, {8, 4}};
+  const int which[2] = {TC_TUNE_PREDICT, TC_TUNE_CORRECT};
+  for(int q = 0; q < 2; q++){
+    int kb = start[q][0], ty = start[q][1];
+    for(;;){
+      const long long nb = nbx * ((s->p.ny + ty - 1) / ty) * ((s->p.nz + kb - 1) / kb);
+      if(nb >= want) break;
+      if(kb > 1) kb /= 2;
+      else if(ty > 1) ty /= 2;
+      else break;
+    }
+    s->tune[which[q]][0] = kb;
+    s->tune[which[q]][1] = ty;
+  }
+}
+
 int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_solver_t ** out){
   *out = NULL;
   if(p->nx < 3 || p->ny < 4 || p->nz < 3) return tc_fail("grid too small: %d x %d x %d", p->nx, p->ny, p->nz);
@@ -350,8 +374,6 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
   tc_solver_t * s = calloc(1, sizeof(tc_solver_t));
   if(!s) return tc_fail("host allocation failed");
   s->p = *p;
-  s->tune[TC_TUNE_PREDICT][0] = 16; s->tune[TC_TUNE_PREDICT][1] = 4;
-  s->tune[TC_TUNE_CORRECT][0] = 8; s->tune[TC_TUNE_CORRECT][1] = 4;
   s->mixed = p->precision == TC_MIXED || p->precision == TC_MIXED_REFINED;
   s->refine = p->precision == TC_MIXED_REFINED;
   s->esz = s->mixed ? sizeof(float) : sizeof(double);
@@ -359,6 +381,7 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
     free(s);
     return tc_fail("hipSetDevice(%d) failed", p->device);
   }
+  default_shapes(s);
   if(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess){
     free(s);
     return tc_fail("hipStreamCreate failed");

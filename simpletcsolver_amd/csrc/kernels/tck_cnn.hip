@@ -68,51 +68,62 @@ struct ConvArgs {
   int relu;
 };
 
+#define CNN_CONV_THREADS 512
+
+/* The block stages the whole kernel W (27 CO CI BF16, at most 54 KiB) in LDS once and its
+ * waves then loop over 16-cell segments, so the B operands come from LDS and only the
+ * activations are read from memory. */
 template <int CI, int CO>
-__global__ void __launch_bounds__(256) tck_cnn_conv(const ConvArgs A){
+__global__ void __launch_bounds__(CNN_CONV_THREADS) tck_cnn_conv(const ConvArgs A){
+  constexpr int NW = 27 * CO * CI;
+  __shared__ __align__(16) unsigned short wl[NW];
+  for(int e = 8 * threadIdx.x; e < NW; e += 8 * CNN_CONV_THREADS) *(uint4 *)(wl + e) = *(const uint4 *)(A.w + e);
+  __syncthreads();
   const int lane = threadIdx.x % TCK_WAVE;
-  const long long wave = (long long)blockIdx.x * 4 + threadIdx.x / TCK_WAVE;
+  const int wpb = CNN_CONV_THREADS / TCK_WAVE;
   const int rows = (A.G.nx + 15) / 16;                 /* 16-cell segments per x row */
   const long long nseg = (long long)rows * A.G.ny * A.G.nz * A.G.nb;
-  if(wave >= nseg) return;
-  const int seg = (int)(wave % rows);
-  const long long yzb = wave / rows;
-  const int y = (int)(yzb % A.G.ny);
-  const int z = (int)((yzb / A.G.ny) % A.G.nz);
-  const int b = (int)(yzb / ((long long)A.G.ny * A.G.nz));
-  const int x = seg * 16 + lane % 16;                  /* A row: this lane's cell */
   const int kq = 4 * (lane / 16);                      /* this lane's 4 K entries */
-  cnn_f4 acc[CO / 16];
-  #pragma unroll
-  for(int t = 0; t < CO / 16; t++) acc[t] = cnn_f4{0.f, 0.f, 0.f, 0.f};
-  for(int tap = 0; tap < 27; tap++){
-    const long long src = x < A.G.nx ? cnn_tap(A.G, x, y, z, b, tap) : -1;
+  for(long long wave = (long long)blockIdx.x * wpb + threadIdx.x / TCK_WAVE; wave < nseg; wave += (long long)gridDim.x * wpb){
+    const int seg = (int)(wave % rows);
+    const long long yzb = wave / rows;
+    const int y = (int)(yzb % A.G.ny);
+    const int z = (int)((yzb / A.G.ny) % A.G.nz);
+    const int b = (int)(yzb / ((long long)A.G.ny * A.G.nz));
+    const int x = seg * 16 + lane % 16;                /* A row: this lane's cell */
+    cnn_f4 acc[CO / 16];
     #pragma unroll
-    for(int c0 = 0; c0 < CI; c0 += 16){
-      /* 4 consecutive BF16 channels: one 8-byte load (rows are 32-byte aligned) */
-      cnn_s4 a = {0, 0, 0, 0};
-      if(src >= 0) a = *(const cnn_s4 *)(A.in + src * CI + c0 + kq);
+    for(int t = 0; t < CO / 16; t++) acc[t] = cnn_f4{0.f, 0.f, 0.f, 0.f};
+    #pragma unroll 3
+    for(int tap = 0; tap < 27; tap++){
+      const long long src = x < A.G.nx ? cnn_tap(A.G, x, y, z, b, tap) : -1;
       #pragma unroll
-      for(int t = 0; t < CO / 16; t++){
-        const cnn_s4 wq = *(const cnn_s4 *)(A.w + ((long long)tap * CO + t * 16 + lane % 16) * CI + c0 + kq);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wq, acc[t], 0, 0, 0);
+      for(int c0 = 0; c0 < CI; c0 += 16){
+        /* 4 consecutive BF16 channels: one 8-byte load (rows are 32-byte aligned) */
+        cnn_s4 a = {0, 0, 0, 0};
+        if(src >= 0) a = *(const cnn_s4 *)(A.in + src * CI + c0 + kq);
+        #pragma unroll
+        for(int t = 0; t < CO / 16; t++){
+          const cnn_s4 wq = *(const cnn_s4 *)(wl + (tap * CO + t * 16 + lane % 16) * CI + c0 + kq);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wq, acc[t], 0, 0, 0);
+        }
       }
     }
-  }
-  /* accumulator: rows (cells) kq .. kq+3 of column (output channel) lane % 16 */
-  #pragma unroll
-  for(int t = 0; t < CO / 16; t++){
-    const int co = t * 16 + lane % 16;
-    if(co >= A.co_store) continue;
-    const float bv = A.bias ? A.bias[co] : 0.f;
+    /* accumulator: rows (cells) kq .. kq+3 of column (output channel) lane % 16 */
     #pragma unroll
-    for(int q = 0; q < 4; q++){
-      const int xc = seg * 16 + kq + q;
-      if(xc >= A.G.nx) continue;
-      const long long v = (long long)xc + (long long)A.G.nx * ((long long)y + (long long)A.G.ny * ((long long)z + (long long)A.G.nz * b));
-      const float val = acc[t][q] + bv;
-      if(A.relu) A.out_bf[v * A.co_store + co] = cnn_bf16(val > 0.f ? val : 0.f);
-      else A.out_f[v * A.co_store + co] = val;
+    for(int t = 0; t < CO / 16; t++){
+      const int co = t * 16 + lane % 16;
+      if(co >= A.co_store) continue;
+      const float bv = A.bias ? A.bias[co] : 0.f;
+      #pragma unroll
+      for(int q = 0; q < 4; q++){
+        const int xc = seg * 16 + kq + q;
+        if(xc >= A.G.nx) continue;
+        const long long v = (long long)xc + (long long)A.G.nx * ((long long)y + (long long)A.G.ny * ((long long)z + (long long)A.G.nz * b));
+        const float val = acc[t][q] + bv;
+        if(A.relu) A.out_bf[v * A.co_store + co] = cnn_bf16(val > 0.f ? val : 0.f);
+        else A.out_f[v * A.co_store + co] = val;
+      }
     }
   }
 }
@@ -332,6 +343,17 @@ __global__ void tck_cnn_scatter(const ScatterArgs A){
 }
 
 /* ------------------------------------------------------------------------------------- */
+static int g_cnn_ncu = 0;
+
+static int cnn_ncu(void){
+  if(g_cnn_ncu == 0){
+    int dev = 0, v = 0;
+    if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    g_cnn_ncu = v;
+  }
+  return g_cnn_ncu;
+}
+
 extern "C" {
 
 int tck_cnn_launch_conv(int nx, int ny, int nz, int nb, const unsigned short * in, int ci,
@@ -341,12 +363,19 @@ int tck_cnn_launch_conv(int nx, int ny, int nz, int nb, const unsigned short * i
   A.G.nx = nx; A.G.ny = ny; A.G.nz = nz; A.G.nb = nb; A.G.nv = (long long)nx * ny * nz * nb;
   A.in = in; A.w = w; A.bias = bias; A.out_bf = out_bf; A.out_f = out_f;
   A.co_store = co_store; A.relu = relu;
+  /* as many blocks as the LDS holds per CU (the staged kernel takes 27 co ci 2 bytes), at
+   * most four, each looping over segments */
   const long long nseg = (long long)((nx + 15) / 16) * ny * nz * nb;
-  const dim3 grd((unsigned)((nseg + 3) / 4));
-  if(ci == 16 && co == 32) hipLaunchKernelGGL((tck_cnn_conv<16, 32>), grd, dim3(256), 0, stream, A);
-  else if(ci == 32 && co == 32) hipLaunchKernelGGL((tck_cnn_conv<32, 32>), grd, dim3(256), 0, stream, A);
-  else if(ci == 32 && co == 16) hipLaunchKernelGGL((tck_cnn_conv<32, 16>), grd, dim3(256), 0, stream, A);
-  else if(ci == 16 && co == 16) hipLaunchKernelGGL((tck_cnn_conv<16, 16>), grd, dim3(256), 0, stream, A);
+  const int wpb = CNN_CONV_THREADS / TCK_WAVE;
+  long long per_cu = (160LL * 1024) / (27LL * co * ci * 2);
+  per_cu = per_cu < 1 ? 1 : (per_cu > 4 ? 4 : per_cu);
+  long long nblk = (nseg + wpb - 1) / wpb;
+  if(nblk > per_cu * cnn_ncu()) nblk = per_cu * cnn_ncu();
+  const dim3 grd((unsigned)nblk), blk(CNN_CONV_THREADS);
+  if(ci == 16 && co == 32) hipLaunchKernelGGL((tck_cnn_conv<16, 32>), grd, blk, 0, stream, A);
+  else if(ci == 32 && co == 32) hipLaunchKernelGGL((tck_cnn_conv<32, 32>), grd, blk, 0, stream, A);
+  else if(ci == 32 && co == 16) hipLaunchKernelGGL((tck_cnn_conv<32, 16>), grd, blk, 0, stream, A);
+  else if(ci == 16 && co == 16) hipLaunchKernelGGL((tck_cnn_conv<16, 16>), grd, blk, 0, stream, A);
   else return (int)hipErrorInvalidValue;
   TCK_CHECK_LAUNCH();
   return 0;

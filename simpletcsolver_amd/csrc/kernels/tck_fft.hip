@@ -435,13 +435,27 @@ static size_t lds_limit(void){
   return g_lds_limit;
 }
 
+static int g_fft_ncu = 0;
+
+static int fft_ncu(void){
+  if(g_fft_ncu == 0){
+    int dev = 0, v = 0;
+    if(hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    g_fft_ncu = v;
+  }
+  return g_fft_ncu;
+}
+
 /* share: tiles sized to leave room for `share` blocks per CU in the LDS (while at least 4
- * transforms, 64 B of contiguous FP64 complex values, remain per row access) */
-static int fft_tile(int M, int cap, size_t csize, int * threads, int share){
+ * transforms, 64 B of contiguous FP64 complex values, remain per row access).  lines: the
+ * transforms of the launch; small launches take narrower tiles, so that there are at least
+ * two blocks per CU */
+static int fft_tile(int M, int cap, size_t csize, int * threads, int share, long long lines){
   int b = 16;
   while(b > 1 && ((size_t)b * (M + 1) * csize > lds_limit() || b * M > FFT_EPT * FFT_MAXTHREADS)) b /= 2;
   while(share > 1 && b > 4 && (size_t)b * (M + 1) * csize * share > lds_limit()) b /= 2;
   while(b * 2 <= cap && b * 2 * M <= FFT_EPT * 64) b *= 2;   /* short transforms: wider tiles */
+  while(b > 1 && lines / b < 2LL * fft_ncu()) b /= 2;
   if(b > cap) b = cap < 1 ? 1 : cap;
   int t = (b * M + FFT_EPT - 1) / FFT_EPT;
   t = (t + 63) / 64 * 64;
@@ -452,7 +466,7 @@ static int fft_tile(int M, int cap, size_t csize, int * threads, int share){
 
 int tck_fft_batch(int M, int cap){
   int t;
-  return fft_tile(M, cap, sizeof(double2), &t, 1);
+  return fft_tile(M, cap, sizeof(double2), &t, 1, 1LL << 40);
 }
 
 /* raises a kernel's dynamic LDS limit once per (kernel, size): later launches, including
@@ -484,7 +498,7 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE);
+  A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE, (long long)g->nx * g->nz);
   A.twM = twM; A.twN = twN;
   A.ld = A.P.M + 1;
   const size_t lds = cs * (size_t)A.ti * A.ld;
@@ -507,7 +521,7 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
   if(make_plan(nz, &A.P) != 0) return (int)hipErrorInvalidValue;
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.tz = fft_tile(nz, 64, cs, &nt, TCK_FFT_ZSHARE);
+  A.tz = fft_tile(nz, 64, cs, &nt, TCK_FFT_ZSHARE, (long long)nx * nky);
   A.ld = nz + 1;
   const size_t lds = cs * (size_t)A.tz * A.ld;
   const dim3 grd((unsigned)((long long)((nx + A.tz - 1) / A.tz) * nky));
@@ -529,7 +543,7 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
-  A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE);
+  A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE, (long long)g->nx * g->nz);
   A.twM = twM; A.twN = twN;
   A.ld = A.P.M + 1;
   const size_t lds = cs * (size_t)A.ti * A.ld;
