@@ -231,3 +231,38 @@ def step_error(pair, states, dt_coarse):
                 num += float(((s1[n] - f1[n]) ** 2).sum())
                 den += float(((f1[n] - f0[n]) ** 2).sum())
     return np.sqrt(num / den)
+
+
+def _main():
+    """time (or profile) the coarse step with an attached network of seeded random weights"""
+    import argparse
+    import time
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--grid", type=int, nargs=3, default=(32, 64, 64))
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--plain", action="store_true", help="without the network")
+    args = ap.parse_args()
+    nx, ny, nz = args.grid
+    lx = 1.0 / 0.714 - 1.0
+    xf, xc = uniform_grid(nx, lx, True)
+    s = Solver(nx, ny, nz, xf, xc, is_curved=True, lengths=(lx, np.pi / 3.0, 2.0 * np.pi * lx),
+               Re=3000.0, implicit=(1, 0, 0), scalar=1, dt_fixed=2e-3)
+    s.ic(seed=1, amp=0.02, t_amp=0.02, nmodes=4, mode_amp=0.1)
+    net = None
+    if not args.plain:
+        net = Cnn(nx, ny, nz, batch=1, seed=1)
+        attach_closure(s, net, np.zeros(5), np.ones(5), 1e-3 * np.ones(4))
+    s.step_async(20)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step_async(args.steps)
+    s.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    print(f"{nx}x{ny}x{nz} {'plain' if args.plain else 'with closure'}: {ms:.3f} ms/step", flush=True)
+    s.close()
+    if net is not None:
+        net.close()
+
+
+if __name__ == "__main__":
+    _main()
