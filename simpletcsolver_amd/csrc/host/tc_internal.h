@@ -89,6 +89,7 @@ struct tc_solver_s {
   int prof_on;
   tc_prof_t prof[TC_NSTAGE];
   int tune[TC_NTUNE][2];
+  int predict_window;     /* 1: the predictor variant with the fields' planes staged in LDS */
   /* learned closure (tc_cnn.c): network, corrections in the field layout (float), inputs */
   tc_cnn_t * closure;
   void * d_corr[TC_NFIELD];

@@ -14,11 +14,12 @@
 #define TC_DTSLOTS 64
 #define TC_DTSTRIDE 16
 /* mixed: the temporaries (increment dlt, srcb, srca, psi) are float and the right-hand sides
- * are evaluated in FP32; otherwise they are double and dlt may be nxt itself */
+ * are evaluated in FP32; otherwise they are double and dlt may be nxt itself.  window: the
+ * variant that stages the fields' planes in LDS (tck_predict_w) */
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     const double * const cur[4], const double * p, double * const nxt[4], void * const dlt[4],
     const void * const srcb[4], void * const srca[4], const float * const corr[4],
-    const double * dt, int rkstep, int final_here, int mixed, int kb, int ty, hipStream_t stream);
+    const double * dt, int rkstep, int final_here, int mixed, int kb, int ty, int window, hipStream_t stream);
 int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     double * ux, double * uy, double * uz, double * p, const void * psi, const double * dt,
     int rkstep, int mode, unsigned long long * dtmin, int mixed, int kb, int ty, hipStream_t stream);

@@ -338,15 +338,16 @@ static int setup_linesolve(tc_solver_t * s){
   return 0;
 }
 
-/* launch shapes of the cell kernels (planes per block kb, rows per block ty): 16 x 4 for the
+/* launch shapes of the cell kernels (planes per block kb, rows per block ty): 32 x 4 for the
  * predictor and 8 x 4 for the correction, shortened until the grid has at least four blocks
- * per CU, so that small grids fill the device (tc_autotune refines them) */
+ * per CU, so that small grids fill the device (tc_autotune refines them).  The predictor
+ * stages the fields' planes in LDS whenever its tiles have 4 rows. */
 static void default_shapes(tc_solver_t * s){
   int ncu = 0;
   if(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->p.device) != hipSuccess || ncu <= 0) ncu = 256;
   const long long want = 4LL * ncu;
   const long long nbx = (s->p.nx + 63) / 64;
-  const int start[2][2] = {{16, 4}, {8, 4}};
+  const int start[2][2] = {{32, 4}, {8, 4}};
   const int which[2] = {TC_TUNE_PREDICT, TC_TUNE_CORRECT};
   for(int q = 0; q < 2; q++){
     int kb = start[q][0], ty = start[q][1];
@@ -360,6 +361,7 @@ static void default_shapes(tc_solver_t * s){
     s->tune[which[q]][0] = kb;
     s->tune[which[q]][1] = ty;
   }
+  s->predict_window = 1;
 }
 
 int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_solver_t ** out){
@@ -806,7 +808,7 @@ static int predict(tc_solver_t * s, int rkstep){
   const float * corr[4] = {NULL, NULL, NULL, NULL};
   if(s->closure) for(int f = 0; f < 4; f++) corr[f] = (const float *)s->d_corr[f];
   PROF(TC_S_PREDICT, tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, dlt, srcb, srca,
-      corr, s->d_dt, rkstep, !any_implicit, s->mixed, s->tune[TC_TUNE_PREDICT][0], s->tune[TC_TUNE_PREDICT][1], s->stream));
+      corr, s->d_dt, rkstep, !any_implicit, s->mixed, s->tune[TC_TUNE_PREDICT][0], s->tune[TC_TUNE_PREDICT][1], s->predict_window, s->stream));
   if(any_implicit){
     int last = 0;
     for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
@@ -934,7 +936,7 @@ int tc_autotune(tc_solver_t * s, int reps, int * kb_out, int * ty_out, double * 
       for(int q = 0; q <= reps && !rc; q++){
         if(q == 1 && hipEventRecord(e0, s->stream) != hipSuccess) rc = 1;
         if(!rc) rc = tck_launch_predict(&s->g, &s->coef, &s->flags, cur, s->d_p, nxt, dlt, srcb, srca,
-            NULL, s->d_dt, 1, !any_implicit, s->mixed, kbs[a], tys[b], s->stream);
+            NULL, s->d_dt, 1, !any_implicit, s->mixed, kbs[a], tys[b], s->predict_window, s->stream);
       }
       float ms = 0.f;
       if(!rc && (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -1103,6 +1105,10 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
 /* launch shapes of the cell kernels ("predict", "correct"): planes per block and j rows per
  * block; results do not depend on them */
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
+  if(kernel && strcmp(kernel, "predict_window") == 0){
+    s->predict_window = kb ? 1 : 0;
+    return 0;
+  }
   int n = -1;
   if(kernel && strcmp(kernel, "predict") == 0) n = TC_TUNE_PREDICT;
   if(kernel && strcmp(kernel, "correct") == 0) n = TC_TUNE_CORRECT;

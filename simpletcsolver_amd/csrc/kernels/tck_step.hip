@@ -75,14 +75,17 @@ struct PredictArgs {
 /* keeps the loads of one field's section from being hoisted above the previous section's
  * stores: shorter live ranges, fewer registers, higher occupancy */
 #define TCK_SECTION_FENCE() asm volatile("" ::: "memory")
-#define CT(t, ii) (smt[(t) * TCK_TW + (ii) - ib])
+#define CT(t, ii) (smt[(t) * TW + (ii) - ib])
 
 /* body with restrict-qualified pointers, so that the loads of each field section are free
  * to move ahead of the stores of the previous one */
-template <typename R>
+template <typename R, typename PT, int TW>
 __device__ __forceinline__ void predict_body(const PredictArgs & A,
-    const double * __restrict__ ux, const double * __restrict__ uy, const double * __restrict__ uz,
-    const double * __restrict__ t, const double * __restrict__ p,
+    const double * __restrict__ Fux_m, const double * __restrict__ Fux_0, const double * __restrict__ Fux_p,
+    const double * __restrict__ Fuy_m, const double * __restrict__ Fuy_0, const double * __restrict__ Fuy_p,
+    const double * __restrict__ Fuz_m, const double * __restrict__ Fuz_0, const double * __restrict__ Fuz_p,
+    const double * __restrict__ Ft_m, const double * __restrict__ Ft_0, const double * __restrict__ Ft_p,
+    const PT * __restrict__ Fp_m, const PT * __restrict__ Fp_0, const int of, const int fsy,
     R * __restrict__ dxo, R * __restrict__ dyo, R * __restrict__ dzo, R * __restrict__ dto,
     double * __restrict__ nxo, double * __restrict__ nyo, double * __restrict__ nzo, double * __restrict__ nto,
     const R * __restrict__ sbx, const R * __restrict__ sby, const R * __restrict__ sbz,
@@ -109,7 +112,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   /* ---------------- ux, faces i = 2..nx (predict/ux.c) ---------------- */
   if(i >= 2){
     R a = R(0.), gg = R(0.);
-    const R u0 = R(ux[o]), uxm = R(ux[o - 1]), uxp = R(ux[o + 1]);
+    const R u0 = R(Fux_0[of]), uxm = R(Fux_0[of - 1]), uxp = R(Fux_0[of + 1]);
     { /* advection_x :92-124 */
       const R ux_xm = CT(T_AXF, i - 1) * uxm + CT(T_AXF, i) * u0;
       const R ux_xp = CT(T_AXF, i) * u0 + CT(T_AXF, i + 1) * uxp;
@@ -119,24 +122,24 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       a -= CT(T_IJDXF, i) * (+ l * uxm + cc * u0 + u * uxp);
     }
     { /* advection_y :126-159 */
-      const R uy_ym = CT(T_AYC, i - 1) * R(uy[o - 1]) + CT(T_AYC, i) * R(uy[o]);
-      const R uy_yp = CT(T_AYC, i - 1) * R(uy[o - 1 + sy]) + CT(T_AYC, i) * R(uy[o + sy]);
+      const R uy_ym = CT(T_AYC, i - 1) * R(Fuy_0[of - 1]) + CT(T_AYC, i) * R(Fuy_0[of]);
+      const R uy_yp = CT(T_AYC, i - 1) * R(Fuy_0[of - 1 + fsy]) + CT(T_AYC, i) * R(Fuy_0[of + fsy]);
       const R l = - R(0.5) * uy_ym;
       const R u = + R(0.5) * uy_yp;
       const R cc = - l - u;
-      a -= CT(T_IJDXF, i) * (+ l * R(ux[o - sy]) + cc * u0 + u * R(ux[o + sy]));
+      a -= CT(T_IJDXF, i) * (+ l * R(Fux_0[of - fsy]) + cc * u0 + u * R(Fux_0[of + fsy]));
     }
     { /* advection_z :161-192 */
-      const R uz_zm = CT(T_AZC, i - 1) * R(uz[o - 1]) + CT(T_AZC, i) * R(uz[o]);
-      const R uz_zp = CT(T_AZC, i - 1) * R(uz[o - 1 + sz]) + CT(T_AZC, i) * R(uz[o + sz]);
+      const R uz_zm = CT(T_AZC, i - 1) * R(Fuz_0[of - 1]) + CT(T_AZC, i) * R(Fuz_0[of]);
+      const R uz_zp = CT(T_AZC, i - 1) * R(Fuz_p[of - 1]) + CT(T_AZC, i) * R(Fuz_p[of]);
       const R l = - R(0.5) * uz_zm;
       const R u = + R(0.5) * uz_zp;
       const R cc = - l - u;
-      a -= CT(T_IJDXF, i) * (+ l * R(ux[o - sz]) + cc * u0 + u * R(ux[o + sz]));
+      a -= CT(T_IJDXF, i) * (+ l * R(Fux_m[of]) + cc * u0 + u * R(Fux_p[of]));
     }
     { /* advection_1 :194-226 (centrifugal) */
-      const R uy_xm = + R(0.5) * R(uy[o - 1]) + R(0.5) * R(uy[o - 1 + sy]);
-      const R uy_xp = + R(0.5) * R(uy[o]) + R(0.5) * R(uy[o + sy]);
+      const R uy_xm = + R(0.5) * R(Fuy_0[of - 1]) + R(0.5) * R(Fuy_0[of - 1 + fsy]);
+      const R uy_xp = + R(0.5) * R(Fuy_0[of]) + R(0.5) * R(Fuy_0[of + fsy]);
       a += CT(T_IJDXF, i) * (+ CT(T_D1M, i) * uy_xm * uy_xm + CT(T_D1P, i) * uy_xp * uy_xp);
     }
     { /* diffusion_x :228-248 */
@@ -144,42 +147,42 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       if(ix) gg += v; else a += v;
     }
     { /* diffusion_y0 :250-269 */
-      const R v = nu * (+ CT(T_LYU_L, i) * R(ux[o - sy]) + CT(T_LYU_C, i) * u0 + CT(T_LYU_U, i) * R(ux[o + sy]));
+      const R v = nu * (+ CT(T_LYU_L, i) * R(Fux_0[of - fsy]) + CT(T_LYU_C, i) * u0 + CT(T_LYU_U, i) * R(Fux_0[of + fsy]));
       if(iy) gg += v; else a += v;
     }
     { /* diffusion_y1 :271-297; lyx1 (lyx.c:56-95) and lxy (lxy.c:19-49) at j and j+1 */
-      const R lyx_ym = CT(T_L1F, i) * (+ R(0.5) * R(uy[o - 1]) + R(0.5) * R(uy[o]));
-      const R lyx_yp = CT(T_L1F, i) * (+ R(0.5) * R(uy[o - 1 + sy]) + R(0.5) * R(uy[o + sy]));
-      const R lxy_ym = CT(T_IHXF, i) * (- R(uy[o - 1]) + R(uy[o]));
-      const R lxy_yp = CT(T_IHXF, i) * (- R(uy[o - 1 + sy]) + R(uy[o + sy]));
+      const R lyx_ym = CT(T_L1F, i) * (+ R(0.5) * R(Fuy_0[of - 1]) + R(0.5) * R(Fuy_0[of]));
+      const R lyx_yp = CT(T_L1F, i) * (+ R(0.5) * R(Fuy_0[of - 1 + fsy]) + R(0.5) * R(Fuy_0[of + fsy]));
+      const R lxy_ym = CT(T_IHXF, i) * (- R(Fuy_0[of - 1]) + R(Fuy_0[of]));
+      const R lxy_yp = CT(T_IHXF, i) * (- R(Fuy_0[of - 1 + fsy]) + R(Fuy_0[of + fsy]));
       a += CT(T_NUJF, i) * (- CT(T_JHYF, i) * (lyx_ym + lxy_ym) + CT(T_JHYF, i) * (lyx_yp + lxy_yp));
     }
     { /* diffusion_z0 :299-318 */
-      const R v = nu * (+ R(c.lapz_l) * R(ux[o - sz]) + R(c.lapz_c) * u0 + R(c.lapz_u) * R(ux[o + sz]));
+      const R v = nu * (+ R(c.lapz_l) * R(Fux_m[of]) + R(c.lapz_c) * u0 + R(c.lapz_u) * R(Fux_p[of]));
       if(iz) gg += v; else a += v;
     }
     { /* diffusion_z1 :320-342; lxz (lxz.c:10-44) at k and k+1 */
-      const R lxz_zm = CT(T_IHXF, i) * (- R(uz[o - 1]) + R(uz[o]));
-      const R lxz_zp = CT(T_IHXF, i) * (- R(uz[o - 1 + sz]) + R(uz[o + sz]));
+      const R lxz_zm = CT(T_IHXF, i) * (- R(Fuz_0[of - 1]) + R(Fuz_0[of]));
+      const R lxz_zp = CT(T_IHXF, i) * (- R(Fuz_p[of - 1]) + R(Fuz_p[of]));
       a += CT(T_NUJF, i) * (- CT(T_JHZF, i) * lxz_zm + CT(T_JHZF, i) * lxz_zp);
     }
     { /* diffusion_a :344-375; lyy0 (lyy.c:24-54) + lyy1 (lyy.c:56-95) at i-1 and i */
-      const R lyy0_m = CT(T_IHYC, i - 1) * (- R(uy[o - 1]) + R(uy[o - 1 + sy]));
+      const R lyy0_m = CT(T_IHYC, i - 1) * (- R(Fuy_0[of - 1]) + R(Fuy_0[of - 1 + fsy]));
       const R lyy1_m = CT(T_L1C, i - 1) * (+ R(0.5) * uxm + R(0.5) * u0);
-      const R lyy0_p = CT(T_IHYC, i) * (- R(uy[o]) + R(uy[o + sy]));
+      const R lyy0_p = CT(T_IHYC, i) * (- R(Fuy_0[of]) + R(Fuy_0[of + fsy]));
       const R lyy1_p = CT(T_L1C, i) * (+ R(0.5) * u0 + R(0.5) * uxp);
       const R lyy_xm = lyy0_m + lyy1_m;
       const R lyy_xp = lyy0_p + lyy1_p;
       a -= CT(T_N2JF, i) * (+ CT(T_D1M, i) * lyy_xm + CT(T_D1P, i) * lyy_xp);
     }
     /* pressure :377-394 */
-    gg -= CT(T_IHXF, i) * (- R(p[o - 1]) + R(p[o]));
+    gg -= CT(T_IHXF, i) * (- R(Fp_0[of - 1]) + R(Fp_0[of]));
     if(A.cx) a += R(A.cx[kb0 + o]);
     R b = R(0.);
     if(sbx) b = sbx[o];
     const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sax) sax[o] = a;
-    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, ux[o] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, Fux_0[of] + (double)delta, false);
     else dxo[o] = delta;
   }
 
@@ -187,11 +190,11 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   /* ---------------- uy, centres i = 1..nx (predict/uy.c) ---------------- */
   {
     R a = R(0.), gg = R(0.);
-    const R v0 = R(uy[o]), vxm = R(uy[o - 1]), vxp = R(uy[o + 1]);
-    const R vym = R(uy[o - sy]), vyp = R(uy[o + sy]);
-    const R vzm = R(uy[o - sz]), vzp = R(uy[o + sz]);
+    const R v0 = R(Fuy_0[of]), vxm = R(Fuy_0[of - 1]), vxp = R(Fuy_0[of + 1]);
+    const R vym = R(Fuy_0[of - fsy]), vyp = R(Fuy_0[of + fsy]);
+    const R vzm = R(Fuy_m[of]), vzp = R(Fuy_p[of]);
     /* ux at (i, j-1), (i, j), (i+1, j-1), (i+1, j) */
-    const R x00 = R(ux[o - sy]), x01 = R(ux[o]), x10 = R(ux[o + 1 - sy]), x11 = R(ux[o + 1]);
+    const R x00 = R(Fux_0[of - fsy]), x01 = R(Fux_0[of]), x10 = R(Fux_0[of + 1 - fsy]), x11 = R(Fux_0[of + 1]);
     { /* advection_x :93-126 */
       const R ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
       const R ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
@@ -209,8 +212,8 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       a -= CT(T_IJDXC, i) * (+ l * vym + cc * v0 + u * vyp);
     }
     { /* advection_z :158-186 */
-      const R uz_zm = CT(T_AZC, i) * R(uz[o - sy]) + CT(T_AZC, i) * R(uz[o]);
-      const R uz_zp = CT(T_AZC, i) * R(uz[o - sy + sz]) + CT(T_AZC, i) * R(uz[o + sz]);
+      const R uz_zm = CT(T_AZC, i) * R(Fuz_0[of - fsy]) + CT(T_AZC, i) * R(Fuz_0[of]);
+      const R uz_zp = CT(T_AZC, i) * R(Fuz_p[of - fsy]) + CT(T_AZC, i) * R(Fuz_p[of]);
       const R l = - R(0.5) * uz_zm;
       const R u = + R(0.5) * uz_zp;
       const R cc = - l - u;
@@ -251,8 +254,8 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       if(iz) gg += v; else a += v;
     }
     { /* diffusion_z1 :343-365; lyz (lyz.c:10-44) at k and k+1 */
-      const R lyz_zm = CT(T_IHYC, i) * (- R(uz[o - sy]) + R(uz[o]));
-      const R lyz_zp = CT(T_IHYC, i) * (- R(uz[o - sy + sz]) + R(uz[o + sz]));
+      const R lyz_zm = CT(T_IHYC, i) * (- R(Fuz_0[of - fsy]) + R(Fuz_0[of]));
+      const R lyz_zp = CT(T_IHYC, i) * (- R(Fuz_p[of - fsy]) + R(Fuz_p[of]));
       a += CT(T_NUJC, i) * (- CT(T_JHZC, i) * lyz_zm + CT(T_JHZC, i) * lyz_zp);
     }
     { /* diffusion_a :367-400 */
@@ -261,13 +264,13 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       a += CT(T_NUJC, i) * (+ CT(T_DAM, i) * (lyx_xm + lxy_xm) + CT(T_DAP, i) * (lyx_xp + lxy_xp));
     }
     /* pressure :402-419 */
-    gg -= CT(T_IHYC, i) * (- R(p[o - sy]) + R(p[o]));
+    gg -= CT(T_IHYC, i) * (- R(Fp_0[of - fsy]) + R(Fp_0[of]));
     if(A.cy) a += R(A.cy[kb0 + o]);
     R b = R(0.);
     if(sby) b = sby[o];
     const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(say) say[o] = a;
-    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, uy[o] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, Fuy_0[of] + (double)delta, false);
     else dyo[o] = delta;
   }
 
@@ -275,13 +278,13 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   /* ---------------- uz, centres (predict/uz.c) ---------------- */
   {
     R a = R(0.), gg = R(0.);
-    const R w0 = R(uz[o]), wxm = R(uz[o - 1]), wxp = R(uz[o + 1]);
-    const R wym = R(uz[o - sy]), wyp = R(uz[o + sy]);
-    const R wzm = R(uz[o - sz]), wzp = R(uz[o + sz]);
+    const R w0 = R(Fuz_0[of]), wxm = R(Fuz_0[of - 1]), wxp = R(Fuz_0[of + 1]);
+    const R wym = R(Fuz_0[of - fsy]), wyp = R(Fuz_0[of + fsy]);
+    const R wzm = R(Fuz_m[of]), wzp = R(Fuz_p[of]);
     /* ux at (i, k-1), (i, k), (i+1, k-1), (i+1, k) */
-    const R x00 = R(ux[o - sz]), x01 = R(ux[o]), x10 = R(ux[o + 1 - sz]), x11 = R(ux[o + 1]);
+    const R x00 = R(Fux_m[of]), x01 = R(Fux_0[of]), x10 = R(Fux_m[of + 1]), x11 = R(Fux_0[of + 1]);
     /* uy at (j, k-1), (j, k), (j+1, k-1), (j+1, k) */
-    const R y00 = R(uy[o - sz]), y01 = R(uy[o]), y10 = R(uy[o + sy - sz]), y11 = R(uy[o + sy]);
+    const R y00 = R(Fuy_m[of]), y01 = R(Fuy_0[of]), y10 = R(Fuy_m[of + fsy]), y11 = R(Fuy_0[of + fsy]);
     { /* advection_x :88-121 */
       const R ux_xm = CT(T_AXF, i) * x00 + CT(T_AXF, i) * x01;
       const R ux_xp = CT(T_AXF, i + 1) * x10 + CT(T_AXF, i + 1) * x11;
@@ -329,13 +332,13 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       if(iz) gg += v; else a += v;
     }
     /* pressure :301-318 */
-    gg -= R(c.ihz) * (- R(p[o - sz]) + R(p[o]));
+    gg -= R(c.ihz) * (- R(Fp_m[of]) + R(Fp_0[of]));
     if(A.cz) a += R(A.cz[kb0 + o]);
     R b = R(0.);
     if(sbz) b = sbz[o];
     const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(saz) saz[o] = a;
-    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, uz[o] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, Fuz_0[of] + (double)delta, false);
     else dzo[o] = delta;
   }
 
@@ -344,28 +347,28 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   if(A.f.scalar){
     const R kappa = R(c.kappa);
     R a = R(0.), gg = R(0.);
-    const R t0 = R(t[o]), txm = R(t[o - 1]), txp = R(t[o + 1]);
-    const R tym = R(t[o - sy]), typ = R(t[o + sy]);
-    const R tzm = R(t[o - sz]), tzp = R(t[o + sz]);
+    const R t0 = R(Ft_0[of]), txm = R(Ft_0[of - 1]), txp = R(Ft_0[of + 1]);
+    const R tym = R(Ft_0[of - fsy]), typ = R(Ft_0[of + fsy]);
+    const R tzm = R(Ft_m[of]), tzp = R(Ft_p[of]);
     { /* advection_x :88-119 */
-      const R ux_xm = CT(T_JHXF, i) * R(ux[o]);
-      const R ux_xp = CT(T_JHXF, i + 1) * R(ux[o + 1]);
+      const R ux_xm = CT(T_JHXF, i) * R(Fux_0[of]);
+      const R ux_xp = CT(T_JHXF, i + 1) * R(Fux_0[of + 1]);
       const R l = - R(0.5) * ux_xm;
       const R u = + R(0.5) * ux_xp;
       const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * txm + cc * t0 + u * txp);
     }
     { /* advection_y :121-148 */
-      const R uy_ym = CT(T_JHYC, i) * R(uy[o]);
-      const R uy_yp = CT(T_JHYC, i) * R(uy[o + sy]);
+      const R uy_ym = CT(T_JHYC, i) * R(Fuy_0[of]);
+      const R uy_yp = CT(T_JHYC, i) * R(Fuy_0[of + fsy]);
       const R l = - R(0.5) * uy_ym;
       const R u = + R(0.5) * uy_yp;
       const R cc = - l - u;
       a -= CT(T_IJDXC, i) * (+ l * tym + cc * t0 + u * typ);
     }
     { /* advection_z :150-176 */
-      const R uz_zm = CT(T_JHZC, i) * R(uz[o]);
-      const R uz_zp = CT(T_JHZC, i) * R(uz[o + sz]);
+      const R uz_zm = CT(T_JHZC, i) * R(Fuz_0[of]);
+      const R uz_zp = CT(T_JHZC, i) * R(Fuz_p[of]);
       const R l = - R(0.5) * uz_zm;
       const R u = + R(0.5) * uz_zp;
       const R cc = - l - u;
@@ -384,7 +387,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbt) b = sbt[o];
     const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sat) sat[o] = a;
-    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, t[o] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, Ft_0[of] + (double)delta, false);
     else dto[o] = delta;
   }
 }
@@ -410,12 +413,21 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
   const int j = 1 + by * blockDim.y + threadIdx.y;
   const int k0 = 1 + bz * A.tm.kb;
   const int k1 = min(k0 + A.tm.kb, A.g.nz + 1);
+  const double * tf = A.t ? A.t : A.ux;   /* T is read only with the scalar on */
+  const long long pl = A.g.plane;
   for(int k = k0; k < k1; k++){
-    const long long kb0 = (long long)(k - 1) * A.g.plane;
+    const long long kb0 = (long long)(k - 1) * pl;
+    /* field offset within a plane; planes k-1, k, k+1 through three base pointers */
+    int of = (i + TC_XOFF) + A.g.px * j;
+    asm volatile("" : "+v"(of));
     #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
     #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
     #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
-    predict_body<R>(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, RB(A.t), A.p + kb0,
+    predict_body<R, double, TCK_TW>(A, A.ux + kb0, A.ux + kb0 + pl, A.ux + kb0 + 2 * pl,
+        A.uy + kb0, A.uy + kb0 + pl, A.uy + kb0 + 2 * pl,
+        A.uz + kb0, A.uz + kb0 + pl, A.uz + kb0 + 2 * pl,
+        tf + kb0, tf + kb0 + pl, tf + kb0 + 2 * pl,
+        A.p + kb0, A.p + kb0 + pl, of, A.g.px,
         RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
         RB(A.nx_), RB(A.ny_), RB(A.nz_), RB(A.nt),
         RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat),
@@ -424,6 +436,112 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     #undef RBT
     #undef RBC
   }
+}
+
+/* barrier that orders LDS only: global loads issued before it stay in flight and pending
+ * global stores are not waited for */
+__device__ __forceinline__ void tck_lds_barrier(){
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+/* The same predictor with the advected fields and p staged in LDS.  A block's 256 threads
+ * cover a tile of 32 i x 8 j cells and march its kb planes keeping planes k-1, k and k+1 of
+ * ux, uy, uz and T (k-1 and k of p) over the tile plus one halo column and row on each side
+ * (34 x 10 cells) in a ring of plane buffers, so that every plane is read from memory once
+ * per block (1.33 times the tile) instead of once per stencil plane, and the stencils read
+ * LDS.  The next plane's loads are issued into registers before the current plane's
+ * stencils run.  49 KiB of LDS with FP64 coefficient rows, 41 KiB in the mixed step: three
+ * blocks per CU.  Outputs and RK sources are addressed as in tck_predict. */
+#define TCK_WX 32                          /* tile width in i */
+#define TCK_WY 8                           /* tile height in j */
+#define TCK_WTW (TCK_WX + 2)               /* window columns (and coefficient-row pitch) */
+#define TCK_WPLANE ((TCK_WY + 2) * TCK_WTW)
+#define TCK_WPF ((TCK_WPLANE + TCK_BLOCK - 1) / TCK_BLOCK)   /* window elements per thread */
+
+template <typename R>
+__global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) tck_predict_w(const PredictArgs A){
+  __shared__ R smt[T_NPRED * TCK_WTW];
+  __shared__ double wf[4][3][TCK_WPLANE];   /* ux uy uz T: a ring of three planes */
+  __shared__ R wq[2][TCK_WPLANE];           /* p: two planes, in the arithmetic type (the
+                                             * stencils read p only through R()) */
+  int bx, by, bz;
+  tile_of(A.tm, &bx, &by, &bz);
+  const int ib = bx * TCK_WX;             /* i of LDS column 0: the tile starts at i = ib + 1 */
+  const int tid = threadIdx.x + TCK_WAVE * threadIdx.y;
+  for(int q = tid; q < T_NPRED * TCK_WTW; q += TCK_BLOCK){
+    const int t = q / TCK_WTW;
+    const int ii = ib + q % TCK_WTW;
+    smt[q] = ii < A.c.tp ? R(A.c.tab[(long long)t * A.c.tp + ii]) : R(0.);
+  }
+  const int li = tid % TCK_WX, lj = tid / TCK_WX;
+  const int i = 1 + ib + li;
+  const int jw = by * TCK_WY;                                /* j of window row 0 */
+  const int j = 1 + jw + lj;
+  const int k0 = 1 + bz * A.tm.kb;
+  const int k1 = min(k0 + A.tm.kb, A.g.nz + 1);
+  const int nrow = min(TCK_WY + 2, A.g.ny + 2 - jw);         /* rows j <= ny + 1 */
+  const int ncol = min(TCK_WTW, A.g.px - TC_XOFF - ib);      /* columns inside a row */
+  const int nf = A.t ? 4 : 3;
+  const long long pl = A.g.plane;
+  const long long px = A.g.px;
+  const double * fld[5] = {A.ux, A.uy, A.uz, A.t ? A.t : A.ux, A.p};
+  /* this thread's window elements e = tid + q TCK_BLOCK: field offsets within a plane (-1:
+   * outside the field) */
+  long long woff[TCK_WPF];
+  #pragma unroll
+  for(int q = 0; q < TCK_WPF; q++){
+    const int e = tid + q * TCK_BLOCK;
+    const int r = e / TCK_WTW, cc = e - r * TCK_WTW;
+    woff[q] = (e < TCK_WPLANE && r < nrow && cc < ncol) ? (long long)(ib + cc + TC_XOFF) + px * (jw + r) : -1;
+  }
+  double pre[5][TCK_WPF];
+  /* fields f < nf at plane kk, and p at plane kq, into registers */
+  #define TCK_FETCH(kk, kq) \
+    _Pragma("unroll") for(int f = 0; f < 5; f++) \
+      _Pragma("unroll") for(int q = 0; q < TCK_WPF; q++) \
+        pre[f][q] = (woff[q] >= 0 && (f < nf || f == 4)) ? fld[f][woff[q] + pl * (f == 4 ? (kq) : (kk))] : 0.;
+  #define TCK_PUT(slot, pslot) \
+    _Pragma("unroll") for(int q = 0; q < TCK_WPF; q++){ \
+      const int e = tid + q * TCK_BLOCK; \
+      if(e < TCK_WPLANE){ \
+        _Pragma("unroll") for(int f = 0; f < 4; f++) wf[f][slot][e] = pre[f][q]; \
+        wq[pslot][e] = R(pre[4][q]); \
+      } \
+    }
+  int sm = 0, s0 = 1, sp = 2, pm = 0;
+  TCK_FETCH(k0 - 1, k0 - 1);
+  TCK_PUT(sm, pm);
+  TCK_FETCH(k0, k0);
+  TCK_PUT(s0, 1 - pm);          /* p's plane k0 is written again below */
+  TCK_FETCH(k0 + 1, k0);
+  const int of = (li + 1) + TCK_WTW * (lj + 1);
+  const int ft = nf == 4 ? 3 : 0;
+  for(int k = k0; k < k1; k++){
+    TCK_PUT(sp, 1 - pm);        /* fields: plane k+1; p: plane k */
+    tck_lds_barrier();
+    if(k + 1 < k1){ TCK_FETCH(k + 2, k + 1); }
+    const long long kb0 = (long long)(k - 1) * pl;
+    #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
+    #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
+    #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
+    predict_body<R, R, TCK_WTW>(A, wf[0][sm], wf[0][s0], wf[0][sp], wf[1][sm], wf[1][s0], wf[1][sp],
+        wf[2][sm], wf[2][s0], wf[2][sp], wf[ft][sm], wf[ft][s0], wf[ft][sp],
+        wq[pm], wq[1 - pm], of, TCK_WTW,
+        RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
+        RB(A.nx_), RB(A.ny_), RB(A.nz_), RB(A.nt),
+        RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat),
+        i, j, k, kb0, smt, ib);
+    #undef RB
+    #undef RBT
+    #undef RBC
+    tck_lds_barrier();
+    const int tmp = sm; sm = s0; s0 = sp; sp = tmp;
+    pm = 1 - pm;
+  }
+  #undef TCK_FETCH
+  #undef TCK_PUT
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -616,8 +734,9 @@ static int check_shape(int kb, int ty){
 int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_t * f,
     const double * const cur[4], const double * p, double * const nxt[4], void * const dlt[4],
     const void * const srcb[4], void * const srca[4], const float * const corr[4],
-    const double * dt, int rkstep, int final_here, int mixed, int kb, int ty, hipStream_t stream){
+    const double * dt, int rkstep, int final_here, int mixed, int kb, int ty, int window, hipStream_t stream){
   if(check_shape(kb, ty)) return (int)hipErrorInvalidValue;
+  if(window && ty != 4) window = 0;       /* the LDS-window variant goes with 4-row shapes */
   PredictArgs A;
   A.g = *g; A.c = *c; A.f = *f;
   A.ux = cur[0]; A.uy = cur[1]; A.uz = cur[2]; A.t = cur[3]; A.p = p;
@@ -629,8 +748,20 @@ int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
   A.cz = corr ? corr[2] : nullptr; A.ct = corr ? corr[3] : nullptr;
   A.dt = dt; A.rkstep = rkstep; A.final_here = final_here;
   A.tm = cell_tiles(*g, kb, ty);
-  if(mixed) hipLaunchKernelGGL(tck_predict<float>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
-  else hipLaunchKernelGGL(tck_predict<double>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
+  if(window){
+    /* 32 x 8 tiles of 256 threads */
+    A.tm.nbx = (g->nx + TCK_WX - 1) / TCK_WX;
+    A.tm.nby = (g->ny + TCK_WY - 1) / TCK_WY;
+    A.tm.total = (long long)A.tm.nbx * A.tm.nby * A.tm.nbz;
+  }
+  const dim3 grd((unsigned)A.tm.total), blk(TCK_WAVE, window ? TCK_BLOCK / TCK_WAVE : ty);
+  if(window){
+    if(mixed) hipLaunchKernelGGL(tck_predict_w<float>, grd, blk, 0, stream, A);
+    else hipLaunchKernelGGL(tck_predict_w<double>, grd, blk, 0, stream, A);
+  }else{
+    if(mixed) hipLaunchKernelGGL(tck_predict<float>, grd, blk, 0, stream, A);
+    else hipLaunchKernelGGL(tck_predict<double>, grd, blk, 0, stream, A);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }

@@ -31,18 +31,41 @@ def _cmp(o, s, names, case, tol, gauge=()):
     return worst
 
 
+def _predictor(s, window):
+    """window 1: the LDS-window predictor (32 x 8 tiles; taken with 4-row shapes, which small
+    grids do not get by default); 0: the direct one with the grid's default shapes"""
+    s.tune("predict_window", window, 0)
+    if window:
+        s.tune("predict", 2, 4)
+
+
+@pytest.mark.parametrize("window", [0, 1])
 @pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 0, 0), (1, 1, 1)])
-def test_predict_substep_matches_oracle(need_gpu, implicit):
+def test_predict_substep_matches_oracle(need_gpu, implicit, window):
     """One predictor stage: explicit stencils round as the reference's (bitwise for the
-    explicit path); implicit line solves agree to a few ulps."""
+    explicit path); implicit line solves agree to a few ulps.  window: the predictor that
+    stages the fields' planes in LDS (same arithmetic, so the same bits)."""
     case = make_case(nx=16, ny=12, nz=8, implicit=implicit, t_amp=1e-2)
     o, s = oracle_for(case), gpu_for(case)
+    _predictor(s, window)
     dt = 1e-3
     for rk in range(3):
         o.predict(rk, dt)
         s.predict(rk, dt)
         tol = 0.0 if implicit == (0, 0, 0) else 1e-14
         _cmp(o, s, VEL + ("t",), case, tol)
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(70, 24, 12), (33, 8, 6)])
+def test_window_predictor_tiles_match_oracle(need_gpu, nx, ny, nz):
+    """the LDS-window predictor over several 32 x 8 tiles, partial in i and j: bit for bit"""
+    case = make_case(nx=nx, ny=ny, nz=nz, implicit=(0, 0, 0), t_amp=1e-2)
+    o, s = oracle_for(case), gpu_for(case)
+    _predictor(s, 1)
+    for rk in range(3):
+        o.predict(rk, 1e-3)
+        s.predict(rk, 1e-3)
+        _cmp(o, s, VEL + ("t",), case, 0.0)
 
 
 def test_potential_matches_oracle(need_gpu):
@@ -56,16 +79,19 @@ def test_potential_matches_oracle(need_gpu):
     _cmp(o, s, ("psi",), case, 1e-12, gauge=("psi",))
 
 
-@pytest.mark.parametrize("implicit,scalar,curved", [
-    ((0, 0, 0), 0, True),
-    ((1, 0, 0), 1, True),
-    ((1, 1, 1), 1, True),
-    ((1, 0, 0), 1, False),
+@pytest.mark.parametrize("implicit,scalar,curved,window", [
+    ((0, 0, 0), 0, True, 0),
+    ((1, 0, 0), 1, True, 0),
+    ((1, 1, 1), 1, True, 0),
+    ((1, 0, 0), 1, False, 0),
+    ((0, 0, 0), 0, True, 1),
+    ((1, 1, 1), 1, True, 1),
 ])
-def test_ten_steps_match_oracle(need_gpu, implicit, scalar, curved):
+def test_ten_steps_match_oracle(need_gpu, implicit, scalar, curved, window):
     case = make_case(nx=24, ny=16, nz=12, implicit=implicit, scalar=scalar, is_curved=curved,
                      Re=500.0, t_amp=1e-2 if scalar else 0.0)
     o, s = oracle_for(case), gpu_for(case)
+    _predictor(s, window)
     for n in range(10):
         dto = o.integrate()
         dts = s.step()
