@@ -367,27 +367,38 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
   __syncthreads();
   const long long nlines = (long long)g.ny * g.nz;
   const int wpb = TCK_BLOCK / TCK_WAVE;
-  for(long long g0 = (long long)blockIdx.x * wpb; g0 < nlines; g0 += (long long)gridDim.x * wpb){
+  const long long stride = (long long)gridDim.x * wpb;
+  /* the line's increment (and old field) are loaded into registers one line ahead, so the
+   * loads of the next line are in flight while this one is solved */
+  D pd[SEG];
+  double pu[SEG];
+  #define XS_FETCH(ln) { \
+    const long long l_ = (ln); \
+    if(l_ < nlines){ \
+      const long long b_ = tck_idx(g, A.i0, 1 + (int)(l_ % g.ny), 1 + (int)(l_ / g.ny)); \
+      _Pragma("unroll") for(int e = 0; e < SEG; e++){ \
+        const int idx = lane + e * TCK_WAVE; \
+        pd[e] = idx < n ? dd[b_ + idx] : D(0.); \
+        if(ucur) pu[e] = idx < n ? ucur[b_ + idx] : 0.; \
+      } \
+    } }
+  XS_FETCH((long long)blockIdx.x * wpb + w);
+  for(long long g0 = (long long)blockIdx.x * wpb; g0 < nlines; g0 += stride){
     const long long line = g0 + w;
     const bool live = line < nlines;
     const int j = live ? 1 + (int)(line % g.ny) : 1;
     const int k = live ? 1 + (int)(line / g.ny) : 1;
-    const long long base = tck_idx(g, A.i0, j, k);
     double u[SEG];
     if(live){
       #pragma unroll
       for(int e = 0; e < SEG; e++){
         const int idx = lane + e * TCK_WAVE;
-        if(idx < n) sm[stage_pos<SEG>(idx)] = double(dd[base + idx]);
-      }
-      if(ucur){
-        #pragma unroll
-        for(int e = 0; e < SEG; e++){
-          const int idx = lane + e * TCK_WAVE;
-          u[e] = idx < n ? ucur[base + idx] : 0.;
-        }
+        if(idx < n) sm[stage_pos<SEG>(idx)] = double(pd[e]);
+        u[e] = pu[e];
       }
     }
+    XS_FETCH(line + stride);
+    const long long base = tck_idx(g, A.i0, j, k);
     wave_sync();
     /* the table offset is opaque to the compiler, so the reads stay inside the loop */
     int toff = lane * (SEG + 1);
@@ -416,6 +427,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
     }
     wave_sync();
   }
+  #undef XS_FETCH
 }
 
 template <int SEG, typename D>
