@@ -108,6 +108,11 @@ template <typename T> struct dftr<8, T> {
   }
 };
 
+/* padded position of element n of a line in LDS: one spare slot after every 8 elements, so
+ * that the first Stockham pass, which writes elements R apart from consecutive threads, and
+ * the row-wise fills spread over all banks */
+__host__ __device__ __forceinline__ int fpad(int n){ return n + (n >> 3); }
+
 template <int R, typename T>
 __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, int ld, int nb, int Ns,
     const typename cvec<T>::type * __restrict__ tw, T s){
@@ -126,10 +131,10 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
       const int b = bf / per;
       const int jj = bf - b * per;
       const int jm = jj % Ns;
-      const V * src = buf + b * ld + jj;
+      const V * src = buf + b * ld;
       #pragma unroll
       for(int r = 0; r < R; r++){
-        V x = src[r * per];
+        V x = src[fpad(jj + r * per)];
         if(r > 0){
           V w = tw[jm * r * twstep];
           if(s > T(0.)) w.y = - w.y;
@@ -148,9 +153,10 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
       const int b = bf / per;
       const int jj = bf - b * per;
       const int jm = jj % Ns;
-      V * dst = buf + b * ld + (jj - jm) * R + jm;
+      V * dst = buf + b * ld;
+      const int d0 = (jj - jm) * R + jm;
       #pragma unroll
-      for(int r = 0; r < R; r++) dst[r * Ns] = v[q][r];
+      for(int r = 0; r < R; r++) dst[fpad(d0 + r * Ns)] = v[q][r];
     }
   }
   __syncthreads();
@@ -190,8 +196,7 @@ struct YFwdArgs {
   const double * dt;
   int rkstep;
   void * S;                      /* spectrum of the transform type */
-  int nky, ti, ld;               /* ld: LDS row pitch, M + 1 (odd, so the rows of a tile
-                                  * start in different banks) */
+  int nky, ti, ld;               /* ld: LDS row pitch of a padded line (fft_ld) */
   FftPlan P;                     /* length M = ny / 2 */
   const void * twM;              /* exp(-2 pi i t / M) */
   const void * twN;              /* exp(-2 pi i k / ny), k = 0..M */
@@ -234,7 +239,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
           - c.jhzc[i] * A.uz[o] + c.jhzc[i] * A.uz[o + sz]);
       val = prefactor * div;
     }
-    ((T *)(sm + t * A.ld))[j0] = T(val);
+    ((T *)(sm + t * A.ld))[2 * fpad(j0 >> 1) + (j0 & 1)] = T(val);
   }
   __syncthreads();
   lds_fft<T>(sm, A.ld, ti, A.P, twM, T(-1.));
@@ -245,8 +250,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
     const int kk = e / ti;
     const int i = i0 + t;
     if(i > g.nx) continue;
-    const V zk = sm[t * A.ld + (kk == M ? 0 : kk)];
-    const V zc = cconj(sm[t * A.ld + (kk == 0 ? 0 : M - kk)]);
+    const V zk = sm[t * A.ld + fpad(kk == M ? 0 : kk)];
+    const V zc = cconj(sm[t * A.ld + fpad(kk == 0 ? 0 : M - kk)]);
     const V w = twN[kk];
     const V ev = mk2<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
     const V od = mk2<T>(T(0.5) * (zk.x - zc.x), T(0.5) * (zk.y - zc.y));
@@ -284,7 +289,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
     const int t = e % tz;
     const int kz = e / tz;
     const int ii = i0 + t;
-    sm[t * A.ld + kz] = ii < A.nx ? base[ii + kstride * kz] : mk2<T>(T(0.), T(0.));
+    sm[t * A.ld + fpad(kz)] = ii < A.nx ? base[ii + kstride * kz] : mk2<T>(T(0.), T(0.));
   }
   __syncthreads();
   lds_fft<T>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
@@ -292,7 +297,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
     const int t = e % tz;
     const int kz = e / tz;
     const int ii = i0 + t;
-    if(ii < A.nx) base[ii + kstride * kz] = sm[t * A.ld + kz];
+    if(ii < A.nx) base[ii + kstride * kz] = sm[t * A.ld + fpad(kz)];
   }
 }
 
@@ -354,7 +359,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
         v = S[o + (long long)g.nx * kk];
       }
     }
-    sm[t * A.ld + kk] = v;
+    sm[t * A.ld + fpad(kk)] = v;
   }
   __syncthreads();
   /* pairs (k, M-k), k = 1..M/2, in place */
@@ -363,14 +368,14 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     const int t = e % ti;
     const int k1 = 1 + e / ti;
     const int k2 = M - k1;
-    const V xa = sm[t * A.ld + k1];
-    const V xb = sm[t * A.ld + k2];
+    const V xa = sm[t * A.ld + fpad(k1)];
+    const V xb = sm[t * A.ld + fpad(k2)];
     const V z1 = c2r_pre(xa, xb, twN[k1]);
     if(k2 != k1){
       const V z2 = c2r_pre(xb, xa, twN[k2]);
-      sm[t * A.ld + k2] = z2;
+      sm[t * A.ld + fpad(k2)] = z2;
     }
-    sm[t * A.ld + k1] = z1;
+    sm[t * A.ld + fpad(k1)] = z1;
   }
   __syncthreads();
   lds_fft<T>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
@@ -380,7 +385,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     const int jj = e / ti;
     const int i = i0 + t;
     if(i > g.nx) continue;
-    const T v = ((const T *)(sm + t * A.ld))[jj];
+    const T v = ((const T *)(sm + t * A.ld))[2 * fpad(jj >> 1) + (jj & 1)];
     tck_store_halo(psi, g, i, jj + 1, k, v, true);
   }
 }
@@ -450,10 +455,18 @@ static int fft_ncu(void){
  * transforms, 64 B of contiguous FP64 complex values, remain per row access).  lines: the
  * transforms of the launch; small launches take narrower tiles, so that there are at least
  * two blocks per CU */
+/* LDS pitch of one padded line of M elements, odd so that the rows of a tile start in
+ * different banks */
+static int fft_ld(int M){
+  const int ld = fpad(M - 1) + 1;
+  return ld | 1;
+}
+
 static int fft_tile(int M, int cap, size_t csize, int * threads, int share, long long lines){
   int b = 16;
-  while(b > 1 && ((size_t)b * (M + 1) * csize > lds_limit() || b * M > FFT_EPT * FFT_MAXTHREADS)) b /= 2;
-  while(share > 1 && b > 4 && (size_t)b * (M + 1) * csize * share > lds_limit()) b /= 2;
+  const size_t lb = (size_t)fft_ld(M) * csize;
+  while(b > 1 && ((size_t)b * lb > lds_limit() || b * M > FFT_EPT * FFT_MAXTHREADS)) b /= 2;
+  while(share > 1 && b > 4 && (size_t)b * lb * share > lds_limit()) b /= 2;
   while(b * 2 <= cap && b * 2 * M <= FFT_EPT * 64) b *= 2;   /* short transforms: wider tiles */
   while(b > 1 && lines / b < 2LL * fft_ncu()) b /= 2;
   if(b > cap) b = cap < 1 ? 1 : cap;
@@ -500,7 +513,7 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   int nt;
   A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE, (long long)g->nx * g->nz);
   A.twM = twM; A.twN = twN;
-  A.ld = A.P.M + 1;
+  A.ld = fft_ld(A.P.M);
   const size_t lds = cs * (size_t)A.ti * A.ld;
   const dim3 grd((unsigned)((long long)((g->nx + A.ti - 1) / A.ti) * g->nz));
   if(mixed){
@@ -522,7 +535,7 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
   A.tz = fft_tile(nz, 64, cs, &nt, TCK_FFT_ZSHARE, (long long)nx * nky);
-  A.ld = nz + 1;
+  A.ld = fft_ld(nz);
   const size_t lds = cs * (size_t)A.tz * A.ld;
   const dim3 grd((unsigned)((long long)((nx + A.tz - 1) / A.tz) * nky));
   if(mixed){
@@ -545,7 +558,7 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
   int nt;
   A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE, (long long)g->nx * g->nz);
   A.twM = twM; A.twN = twN;
-  A.ld = A.P.M + 1;
+  A.ld = fft_ld(A.P.M);
   const size_t lds = cs * (size_t)A.ti * A.ld;
   const dim3 grd((unsigned)((long long)((g->nx + A.ti - 1) / A.ti) * g->nz));
   if(mixed){
