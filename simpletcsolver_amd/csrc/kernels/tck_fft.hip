@@ -301,6 +301,70 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   }
 }
 
+/* Persistent form: each block loops over tiles and issues the loads of the first PF of its
+ * elements of the next tile before the current tile's transform, so that part of the HBM
+ * traffic overlaps the LDS passes (the remaining elements are loaded when the tile starts). */
+#ifndef TCK_ZFFT_PF
+#define TCK_ZFFT_PF 8            /* elements per thread loaded one tile ahead (FP64 only: the
+                                  * FP32 form, at 1024 threads per block, would spill) */
+#endif
+#define FFT_BOUND(T) (sizeof(T) == 8 ? 512 : FFT_MAXTHREADS)
+
+__device__ __forceinline__ void fft_lds_barrier(){
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename T, int PF>
+__global__ void __launch_bounds__(FFT_BOUND(T)) tck_zfft_p(const ZArgs A){
+  typedef typename cvec<T>::type V;
+  extern __shared__ __align__(16) unsigned char fft_smem[];
+  V * sm = (V *)fft_smem;
+  const int tz = A.tz, nz = A.nz, nt = blockDim.x, nel = tz * nz;
+  const int ntx = (A.nx + tz - 1) / tz;
+  const long long ntiles = (long long)ntx * A.nky;
+  const long long kstride = (long long)A.nx * A.nky;
+  V pre[PF];
+  long long tile = blockIdx.x;
+  #define ZF_ELEM(tl, e, out) { \
+    const int t_ = (e) % tz, kz_ = (e) / tz; \
+    const int ii_ = (int)((tl) % ntx) * tz + t_; \
+    const V * b_ = (const V *)A.S + (long long)A.nx * ((tl) / ntx); \
+    out = ((e) < nel && ii_ < A.nx) ? b_[ii_ + kstride * kz_] : mk2<T>(T(0.), T(0.)); }
+  if(tile < ntiles){
+    #pragma unroll
+    for(int q = 0; q < PF; q++){ ZF_ELEM(tile, threadIdx.x + q * nt, pre[q]); }
+  }
+  for(; tile < ntiles; tile += gridDim.x){
+    #pragma unroll
+    for(int q = 0; q < PF; q++){
+      const int e = threadIdx.x + q * nt;
+      if(e < nel) sm[(e % tz) * A.ld + fpad(e / tz)] = pre[q];
+    }
+    for(int e = threadIdx.x + PF * nt; e < nel; e += nt){
+      V v;
+      ZF_ELEM(tile, e, v);
+      sm[(e % tz) * A.ld + fpad(e / tz)] = v;
+    }
+    fft_lds_barrier();
+    if(tile + gridDim.x < ntiles){
+      #pragma unroll
+      for(int q = 0; q < PF; q++){ ZF_ELEM(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
+    }
+    lds_fft<T>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
+    const int i0 = (int)(tile % ntx) * tz;
+    V * base = (V *)A.S + (long long)A.nx * (tile / ntx);
+    for(int e = threadIdx.x; e < nel; e += nt){
+      const int t = e % tz;
+      const int kz = e / tz;
+      if(i0 + t < A.nx) base[i0 + t + kstride * kz] = sm[t * A.ld + fpad(kz)];
+    }
+    fft_lds_barrier();
+  }
+  #undef ZF_ELEM
+}
+
 /* ------------------------------------------------------------------------------------- */
 struct YInvArgs {
   tc_geom_t g;
@@ -537,7 +601,28 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
   A.tz = fft_tile(nz, 64, cs, &nt, TCK_FFT_ZSHARE, (long long)nx * nky);
   A.ld = fft_ld(nz);
   const size_t lds = cs * (size_t)A.tz * A.ld;
-  const dim3 grd((unsigned)((long long)((nx + A.tz - 1) / A.tz) * nky));
+  const long long ntiles = (long long)((nx + A.tz - 1) / A.tz) * nky;
+#if TCK_ZFFT_PF > 0
+  if(!mixed && nt <= FFT_BOUND(double)){
+    const void * fn = (const void *)tck_zfft_p<double, TCK_ZFFT_PF>;
+    if(set_lds(fn, lds)) return (int)hipErrorInvalidValue;
+    /* resident blocks per CU, queried once per (threads, LDS) so that launches recorded into
+     * a graph make no query */
+    static int c_nt = 0, c_per = 0;
+    static size_t c_lds = 0;
+    int per = 0, ncu = fft_ncu();
+    if(c_nt == nt && c_lds == lds) per = c_per;
+    else{
+      if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
+      c_nt = nt; c_lds = lds; c_per = per;
+    }
+    const long long nb = ntiles < (long long)per * ncu ? ntiles : (long long)per * ncu;
+    hipLaunchKernelGGL((tck_zfft_p<double, TCK_ZFFT_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
+    TCK_CHECK_LAUNCH();
+    return 0;
+  }
+#endif
+  const dim3 grd((unsigned)ntiles);
   if(mixed){
     if(set_lds((const void *)tck_zfft<float>, lds)) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(tck_zfft<float>, grd, dim3(nt), lds, stream, A);
