@@ -616,10 +616,13 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
       if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
       c_nt = nt; c_lds = lds; c_per = per;
     }
-    const long long nb = ntiles < (long long)per * ncu ? ntiles : (long long)per * ncu;
-    hipLaunchKernelGGL((tck_zfft_p<double, TCK_ZFFT_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
-    TCK_CHECK_LAUNCH();
-    return 0;
+    /* persistence pays only when every block has several tiles to loop over */
+    const long long nb = (long long)per * ncu;
+    if(ntiles >= 4 * nb){
+      hipLaunchKernelGGL((tck_zfft_p<double, TCK_ZFFT_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
+      TCK_CHECK_LAUNCH();
+      return 0;
+    }
   }
 #endif
   const dim3 grd((unsigned)ntiles);

@@ -89,7 +89,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     R * __restrict__ dxo, R * __restrict__ dyo, R * __restrict__ dzo, R * __restrict__ dto,
     double * __restrict__ nxo, double * __restrict__ nyo, double * __restrict__ nzo, double * __restrict__ nto,
     const R * __restrict__ sbx, const R * __restrict__ sby, const R * __restrict__ sbz,
-    const R * __restrict__ sbt,
+    const R * __restrict__ sbt, const R * sbv,
     R * __restrict__ sax, R * __restrict__ say, R * __restrict__ saz, R * __restrict__ sat,
     const int i, const int j, const int k, const long long kb0, const R * smt, const int ib){
   const tc_geom_t & g = A.g;
@@ -179,7 +179,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     gg -= CT(T_IHXF, i) * (- R(Fp_0[of - 1]) + R(Fp_0[of]));
     if(A.cx) a += R(A.cx[kb0 + o]);
     R b = R(0.);
-    if(sbx) b = sbx[o];
+    if(sbx) b = sbv ? sbv[0] : sbx[o];
     const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sax) sax[o] = a;
     if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, Fux_0[of] + (double)delta, false);
@@ -267,7 +267,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     gg -= CT(T_IHYC, i) * (- R(Fp_0[of - fsy]) + R(Fp_0[of]));
     if(A.cy) a += R(A.cy[kb0 + o]);
     R b = R(0.);
-    if(sby) b = sby[o];
+    if(sby) b = sbv ? sbv[1] : sby[o];
     const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(say) say[o] = a;
     if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, Fuy_0[of] + (double)delta, false);
@@ -335,7 +335,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     gg -= R(c.ihz) * (- R(Fp_m[of]) + R(Fp_0[of]));
     if(A.cz) a += R(A.cz[kb0 + o]);
     R b = R(0.);
-    if(sbz) b = sbz[o];
+    if(sbz) b = sbv ? sbv[2] : sbz[o];
     const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(saz) saz[o] = a;
     if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, Fuz_0[of] + (double)delta, false);
@@ -384,7 +384,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     }
     if(A.ct) a += R(A.ct[kb0 + o]);
     R b = R(0.);
-    if(sbt) b = sbt[o];
+    if(sbt) b = sbv ? sbv[3] : sbt[o];
     const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sat) sat[o] = a;
     if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, Ft_0[of] + (double)delta, false);
@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
         A.p + kb0, A.p + kb0 + pl, of, A.g.px,
         RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
         RB(A.nx_), RB(A.ny_), RB(A.nz_), RB(A.nt),
-        RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat),
+        RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), (const R *)nullptr, RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat),
         i, j, k, kb0, smt, ib);
     #undef RB
     #undef RBT
@@ -497,11 +497,21 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     woff[q] = (e < TCK_WPLANE && r < nrow && cc < ncol) ? (long long)(ib + cc + TC_XOFF) + px * (jw + r) : -1;
   }
   double pre[5][TCK_WPF];
-  /* fields f < nf at plane kk, and p at plane kq, into registers */
-  #define TCK_FETCH(kk, kq) \
+  /* the cell's RK sources of the previous substep (substeps 1 and 2): loaded with the next
+   * plane in the mixed step; the FP64 form has no registers to spare for them and the body
+   * loads them where they are used */
+  constexpr bool PFS = sizeof(R) == 4;
+  const bool inside = i <= A.g.nx && j <= A.g.ny;
+  const long long ocell = (long long)(i + TC_XOFF) + px * j;
+  const R * sbp[4] = {(const R *)A.sbx, (const R *)A.sby, (const R *)A.sbz, (const R *)A.sbt};
+  R psb[4] = {R(0.), R(0.), R(0.), R(0.)};
+  /* fields f < nf at plane kk and p at plane kq into registers; sources at plane ks */
+  #define TCK_FETCH(kk, kq, ks) \
     _Pragma("unroll") for(int f = 0; f < 5; f++) \
       _Pragma("unroll") for(int q = 0; q < TCK_WPF; q++) \
-        pre[f][q] = (woff[q] >= 0 && (f < nf || f == 4)) ? fld[f][woff[q] + pl * (f == 4 ? (kq) : (kk))] : 0.;
+        pre[f][q] = (woff[q] >= 0 && (f < nf || f == 4)) ? fld[f][woff[q] + pl * (f == 4 ? (kq) : (kk))] : 0.; \
+    if(PFS){ _Pragma("unroll") for(int f = 0; f < 4; f++) \
+      psb[f] = (inside && sbp[f]) ? sbp[f][ocell + pl * (ks)] : R(0.); }
   #define TCK_PUT(slot, pslot) \
     _Pragma("unroll") for(int q = 0; q < TCK_WPF; q++){ \
       const int e = tid + q * TCK_BLOCK; \
@@ -511,17 +521,21 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
       } \
     }
   int sm = 0, s0 = 1, sp = 2, pm = 0;
-  TCK_FETCH(k0 - 1, k0 - 1);
+  TCK_FETCH(k0 - 1, k0 - 1, k0);
   TCK_PUT(sm, pm);
-  TCK_FETCH(k0, k0);
+  TCK_FETCH(k0, k0, k0);
   TCK_PUT(s0, 1 - pm);          /* p's plane k0 is written again below */
-  TCK_FETCH(k0 + 1, k0);
+  TCK_FETCH(k0 + 1, k0, k0);
   const int of = (li + 1) + TCK_WTW * (lj + 1);
   const int ft = nf == 4 ? 3 : 0;
   for(int k = k0; k < k1; k++){
     TCK_PUT(sp, 1 - pm);        /* fields: plane k+1; p: plane k */
+    R sbv[4];
+    #pragma unroll
+    for(int f = 0; f < 4; f++) sbv[f] = psb[f];
     tck_lds_barrier();
-    if(k + 1 < k1){ TCK_FETCH(k + 2, k + 1); }
+    if(k + 1 < k1){ TCK_FETCH(k + 2, k + 1, k + 1); }
+
     const long long kb0 = (long long)(k - 1) * pl;
     #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
     #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
@@ -531,8 +545,8 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
         wq[pm], wq[1 - pm], of, TCK_WTW,
         RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
         RB(A.nx_), RB(A.ny_), RB(A.nz_), RB(A.nt),
-        RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat),
-        i, j, k, kb0, smt, ib);
+        RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), PFS ? (const R *)sbv : (const R *)nullptr,
+        RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat), i, j, k, kb0, smt, ib);
     #undef RB
     #undef RBT
     #undef RBC
