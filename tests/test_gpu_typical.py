@@ -6,6 +6,7 @@ fields).  The flow settles into steady Taylor vortices; the logged energy inject
 dissipation, normalised by the laminar torque as the reference's balance_main.py does, give
 the Nusselt number that the documentation compares with Ostilla et al. (JFM 2013), 1.1375.
 Two initial-field seeds reach the same state."""
+import json
 import os
 import subprocess
 
@@ -16,18 +17,21 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "simpletcsolver_amd", "tcsolver")
-LENGTHS = (1.0, 0.39269908169872414, 2.0)
+with open(os.path.join(ROOT, "tests", "golden", "typical_case.json")) as fh:
+    CASE = json.load(fh)
+LENGTHS = tuple(CASE["lengths"])
 
 
 def _run(tmp, seed, timemax):
     from simpletcsolver_amd import icdir
-    icdir.write(os.path.join(tmp, "ic"), True, LENGTHS, (64, 8, 32), seed=seed)
+    icdir.write(os.path.join(tmp, "ic"), CASE["is_curved"], LENGTHS, tuple(CASE["glsizes"]), seed=seed)
     for sub in ("log", "save", "stat"):
         os.makedirs(os.path.join(tmp, "output", sub), exist_ok=True)
     env = dict(os.environ)
     env.update({"timemax": repr(timemax), "wtimemax": "600", "log_rate": "1.0", "save_rate": "1000",
-                "save_after": "0", "stat_rate": "0.1", "stat_after": "1000", "coef_dt_adv": "1.",
-                "coef_dt_dif": "1.", "Re": "80.", "Pr": "1."})
+                "save_after": "0", "stat_rate": "0.1", "stat_after": "1000",
+                "coef_dt_adv": repr(CASE["coef_dt_adv"]), "coef_dt_dif": repr(CASE["coef_dt_dif"]),
+                "Re": repr(CASE["Re"]), "Pr": repr(CASE["Pr"])})
     print(f"tcsolver typical case, seed {seed}, timemax {timemax}", flush=True)
     r = subprocess.run([EXE, "ic"], cwd=tmp, env=env, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -41,7 +45,7 @@ def _run(tmp, seed, timemax):
 def test_typical_case_nusselt(need_gpu, tmp_path, seed):
     from simpletcsolver_amd import icdir
     inj, dis, div = _run(str(tmp_path), seed, 200.0)
-    f = icdir.laminar_torque_factor(80.0, LENGTHS)
+    f = icdir.laminar_torque_factor(CASE["Re"], LENGTHS)
     late = inj[:, 0] > 150.0
     nu_i = inj[late, 1].mean() / f
     nu_d = dis[late, 1].mean() / f
@@ -49,7 +53,7 @@ def test_typical_case_nusselt(need_gpu, tmp_path, seed):
           f"max div {div[:, 1].max():.1e}")
     # the discretisation's own value at this resolution is 1.1345 (0.26 % below the reference
     # value the documentation draws)
-    assert abs(nu_i - 1.1375) <= 5e-3 * 1.1375
+    assert abs(nu_i - CASE["nusselt"]) <= 5e-3 * CASE["nusselt"]
     # energy balance of the steady state: injection equals dissipation (velocity and scalar)
     assert abs(nu_i - nu_d) <= 1e-5 * nu_i
     assert abs(inj[late, 2].mean() - dis[late, 2].mean()) <= 1e-5 * abs(inj[late, 2].mean())
