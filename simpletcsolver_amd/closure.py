@@ -119,12 +119,18 @@ class Pair:
             f.pop("t", None)
         return f
 
-    def coarse_step(self, fc, dt, steps=1):
-        """the coarse solver from fields fc, `steps` steps of fixed dt; its fields after"""
+    def coarse_step(self, fc, dt, steps=1, torque=None):
+        """the coarse solver from fields fc, `steps` steps of fixed dt; its fields after.
+        torque: a list that receives the energy injection (wall torque) of the first step"""
         self.coarse.set_dt_params(0.5, 0.5, dt)
         self.coarse.load_fields(fc)
-        for _ in range(steps):
+        for n in range(steps):
+            if n == 0 and torque is not None:
+                self.coarse.arm_diagnostics(True)
             self.coarse.step()
+            if n == 0 and torque is not None:
+                torque.append(float(self.coarse.injection()[0]))
+                self.coarse.arm_diagnostics(False)
         g = self.coarse.fields()
         g.pop("psi", None)
         if not self.scalar:
@@ -132,11 +138,13 @@ class Pair:
         return g
 
 
-def generate(pair, n_samples, spacing, dt_coarse, states=None, log=None):
+def generate(pair, n_samples, spacing, dt_coarse, states=None, log=None, torques=None):
     """n_samples (inputs, targets) pairs from the fine run, `spacing` coarse steps apart.
     The fine solver runs at dt_coarse / r; returns x [n][nz][ny][nx][5] and the target
     corrections y [n][nz][ny][nx][4] (FP32).  states: a list that receives the filtered
-    (start, end) fields of every sample."""
+    (start, end) fields of every sample; torques: a list that receives, per sample, the
+    energy injection (wall torque) of the fine step and of the coarse step from the filtered
+    field, the data of a torque calibration of the coarse grid's wall stencil."""
     r = pair.r
     pair.fine.set_dt_params(0.5, 0.5, dt_coarse / r)
     xs, ys = [], []
@@ -144,10 +152,18 @@ def generate(pair, n_samples, spacing, dt_coarse, states=None, log=None):
         if log and m % 8 == 0:
             log(f"sample {m} of {n_samples}")
         f0 = box_filter(pair.fine_state(), r)
-        for _ in range(r):
+        for q in range(r):
+            if q == 0 and torques is not None:
+                pair.fine.arm_diagnostics(True)
             pair.fine.step()
+            if q == 0 and torques is not None:
+                tf = float(pair.fine.injection()[0])
+                pair.fine.arm_diagnostics(False)
         f1 = box_filter(pair.fine_state(), r)
-        s1 = pair.coarse_step(f0, dt_coarse)
+        tc = [] if torques is not None else None
+        s1 = pair.coarse_step(f0, dt_coarse, torque=tc)
+        if torques is not None:
+            torques.append((tf, tc[0]))
         corr = {n: (f1[n] - s1[n]) / dt_coarse for n in CORR_FIELDS if n in f1}
         if states is not None:
             states.append((f0, f1))

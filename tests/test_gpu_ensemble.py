@@ -89,3 +89,28 @@ def test_ensemble_many_steps_in_one_call(need_gpu):
     for s in solos:
         s.close()
     ens.close()
+
+
+def test_ensemble_of_mixed_precision_members(need_gpu):
+    """members of either precision side by side: each equals its solo run bit for bit"""
+    from simpletcsolver_amd import Ensemble
+    specs = _specs()[:3]
+    specs[0][0]["precision"] = "mixed"
+    specs[2][0]["precision"] = "mixed_refined"
+    ens = Ensemble([sp for sp, _ in specs])
+    for m, (_, f) in enumerate(specs):
+        ens.members[m].load_fields(f)
+    solos = [_solo(sp, f) for sp, f in specs]
+    ens.step(4)
+    ens.synchronize()
+    for s in solos:
+        for _ in range(4):
+            s.step()
+    for m, s in enumerate(solos):
+        got, want = ens.members[m].fields(), s.fields()
+        for n in ("ux", "uy", "uz", "p", "t"):
+            assert np.array_equal(got[n], want[n]), (m, n)
+    assert ens.members[2].divergence() <= 1e-12
+    for s in solos:
+        s.close()
+    ens.close()
