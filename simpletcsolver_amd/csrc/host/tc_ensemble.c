@@ -1,10 +1,11 @@
 /*
  * tc_ensemble.c -- a batch of independent flows of uneven sizes advanced together.
  *
- * The reference runs one flow per process.  Here n solvers share one HIP stream; after one
- * eager step, each RK substep of the whole batch (dt and factor tables in the first, then
- * predict, Poisson solve and correction of every member) is recorded once into a hipGraph, so
- * a step of the batch is three graph launches whatever n is.  The members' double buffers
+ * The reference runs one flow per process.  Here n solvers are spread over a few HIP streams
+ * (lanes); after one eager step, each RK substep of the whole batch (dt and factor tables in
+ * the first, then predict, Poisson solve and correction of every member) is recorded once
+ * into a hipGraph whose lanes are independent branches, so a step of the batch is three graph
+ * launches whatever n is, and members of different lanes may run concurrently.  The members' double buffers
  * alternate from step to step, so the substeps of two consecutive steps are recorded (six
  * graphs) and replayed by step parity; the members' host-side bookkeeping advances with each
  * replayed step exactly as an eager step would advance it.
@@ -19,10 +20,15 @@
 #include "tcstep.h"
 #include "tc_internal.h"
 
+#define TC_ENS_LANES 4   /* streams the members are spread over (hardware queues per process) */
+
 struct tc_ensemble_s {
   int n;
   tc_solver_t ** s;
-  hipStream_t stream;
+  hipStream_t stream;         /* root of the recorded graphs and their launches */
+  int nlanes;
+  hipStream_t lane[TC_ENS_LANES];   /* member m works on lane m % nlanes */
+  hipEvent_t fork, join[TC_ENS_LANES];
   int device;
   int recorded;
   long long steps;            /* steps taken since the recording (parity of the replay) */
@@ -44,12 +50,22 @@ int tc_ensemble_create(int n, const tc_params_t * params, const double * const *
     free(e);
     return tc_fail("tc_ensemble_create: stream creation failed");
   }
+  e->nlanes = n < TC_ENS_LANES ? n : TC_ENS_LANES;
+  int bad = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess;
+  for(int l = 0; l < e->nlanes && !bad; l++){
+    bad = hipStreamCreateWithFlags(&e->lane[l], hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&e->join[l], hipEventDisableTiming) != hipSuccess;
+  }
+  if(bad){
+    tc_ensemble_destroy(e);
+    return tc_fail("tc_ensemble_create: stream or event creation failed");
+  }
   for(int m = 0; m < n; m++){
     if(params[m].device != e->device){
       tc_ensemble_destroy(e);
       return tc_fail("tc_ensemble_create: all members must use device %d", e->device);
     }
-    if(tc_create(&params[m], xf[m], xc[m], &e->s[m]) || tc_share_stream(e->s[m], e->stream)){
+    if(tc_create(&params[m], xf[m], xc[m], &e->s[m]) || tc_share_stream(e->s[m], e->lane[m % e->nlanes])){
       tc_ensemble_destroy(e);
       return 1;
     }
@@ -76,12 +92,23 @@ static void drop_graphs(tc_ensemble_t * e){
   e->recorded = 0;
 }
 
+static int sync_all(tc_ensemble_t * e){
+  if(e->stream) HIPOK(hipStreamSynchronize(e->stream));
+  for(int l = 0; l < e->nlanes; l++) if(e->lane[l]) HIPOK(hipStreamSynchronize(e->lane[l]));
+  return 0;
+}
+
 int tc_ensemble_destroy(tc_ensemble_t * e){
   if(!e) return 0;
   hipSetDevice(e->device);
-  if(e->stream) hipStreamSynchronize(e->stream);
+  sync_all(e);
   drop_graphs(e);
   for(int m = 0; m < e->n; m++) tc_destroy(e->s[m]);
+  for(int l = 0; l < e->nlanes; l++){
+    if(e->lane[l]) hipStreamDestroy(e->lane[l]);
+    if(e->join[l]) hipEventDestroy(e->join[l]);
+  }
+  if(e->fork) hipEventDestroy(e->fork);
   if(e->stream) hipStreamDestroy(e->stream);
   free(e->s);
   free(e);
@@ -108,7 +135,7 @@ int tc_ensemble_autotune(tc_ensemble_t * e, int reps, int * kb, int * ty){
   }
   /* recorded graphs hold the old shapes */
   if(e->recorded){
-    HIPOK(hipStreamSynchronize(e->stream));
+    if(sync_all(e)) return 1;
     drop_graphs(e);
   }
   return 0;
@@ -118,15 +145,23 @@ int tc_ensemble_autotune(tc_ensemble_t * e, int reps, int * kb, int * ty){
  * two steps recorded; nothing runs while recording, and the members' host state returns to
  * where it was after the two recorded steps (even parity) */
 static int record(tc_ensemble_t * e){
+  if(sync_all(e)) return 1;
   for(int m = 0; m < e->n; m++){
     if(tc_enqueue_step(e->s[m])) return 1;
   }
-  HIPOK(hipStreamSynchronize(e->stream));
+  if(sync_all(e)) return 1;
   for(int p = 0; p < 2; p++){
     for(int r = 0; r < 3; r++){
+      /* the lanes fork from the root stream and join it again: the members' kernels form
+       * independent branches of the graph */
       HIPOK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
+      int rc = hipEventRecord(e->fork, e->stream) != hipSuccess;
+      for(int l = 0; l < e->nlanes && !rc; l++) rc = hipStreamWaitEvent(e->lane[l], e->fork, 0) != hipSuccess;
       for(int m = 0; m < e->n && !rc; m++) rc = tc_enqueue_substep(e->s[m], r);
+      for(int l = 0; l < e->nlanes && !rc; l++){
+        rc = hipEventRecord(e->join[l], e->lane[l]) != hipSuccess ||
+             hipStreamWaitEvent(e->stream, e->join[l], 0) != hipSuccess;
+      }
       hipGraph_t g = NULL;
       const hipError_t ce = hipStreamEndCapture(e->stream, &g);
       if(rc || ce != hipSuccess){
@@ -155,26 +190,34 @@ int tc_ensemble_step(tc_ensemble_t * e, int nsteps){
     if(record(e)) return 1;   /* includes one eager step */
     left -= 1;
   }
+  if(left < 1) return 0;
+  /* the graphs run after whatever the members' lanes hold, and the lanes continue after the
+   * graphs (reads of a member's fields are then ordered after the replayed steps) */
+  for(int l = 0; l < e->nlanes; l++){
+    HIPOK(hipEventRecord(e->join[l], e->lane[l]));
+    HIPOK(hipStreamWaitEvent(e->stream, e->join[l], 0));
+  }
   for(int q = 0; q < left; q++){
     const int p = (int)(e->steps & 1);
     for(int r = 0; r < 3; r++) HIPOK(hipGraphLaunch(e->exec[p][r], e->stream));
     for(int m = 0; m < e->n; m++) tc_advance_host_state(e->s[m]);
     e->steps += 1;
   }
+  HIPOK(hipEventRecord(e->fork, e->stream));
+  for(int l = 0; l < e->nlanes; l++) HIPOK(hipStreamWaitEvent(e->lane[l], e->fork, 0));
   return 0;
 }
 
 int tc_ensemble_synchronize(tc_ensemble_t * e){
   HIPOK(hipSetDevice(e->device));
-  HIPOK(hipStreamSynchronize(e->stream));
-  return 0;
+  return sync_all(e);
 }
 
 /* a member's fields were replaced from the host: its recorded step is still valid (same
  * buffers), but the parity of the replay must match the member's buffers again */
 int tc_ensemble_invalidate(tc_ensemble_t * e){
   HIPOK(hipSetDevice(e->device));
-  HIPOK(hipStreamSynchronize(e->stream));
+  if(sync_all(e)) return 1;
   drop_graphs(e);
   return 0;
 }
