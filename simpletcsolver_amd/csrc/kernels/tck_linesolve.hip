@@ -421,7 +421,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
         #pragma unroll
         for(int e = 0; e < SEG; e++){
           const int idx = lane + e * TCK_WAVE;
-          if(idx < n) dd[base + idx] = D(sm[stage_pos<SEG>(idx)]);
+          if(idx < n) tck_st(&dd[base + idx], D(sm[stage_pos<SEG>(idx)]));
         }
       }
     }
@@ -642,7 +642,7 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
             const int kk = A.dir == 1 ? other : 1 + row;
             tck_store_halo(uout, g, i, jj, kk, uu[e] + double(x), false);
           }else{
-            d[base + step * row] = D(x);
+            tck_st(&d[base + step * row], D(x));
           }
         }
       }
@@ -654,7 +654,7 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
       const int kk = A.dir == 1 ? other : n;
       tck_store_halo(uout, g, i, jj, kk, ucur[base + step * m] + double(xl), false);
     }else{
-      d[base + step * m] = D(xl);
+      tck_st(&d[base + step * m], D(xl));
     }
   }
 }

@@ -181,9 +181,9 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     R b = R(0.);
     if(sbx) b = sbv ? sbv[0] : sbx[o];
     const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(sax) sax[o] = a;
+    if(sax) tck_st(&sax[o], a);
     if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, Fux_0[of] + (double)delta, false);
-    else dxo[o] = delta;
+    else tck_st(&dxo[o], delta);
   }
 
   TCK_SECTION_FENCE();
@@ -269,9 +269,9 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     R b = R(0.);
     if(sby) b = sbv ? sbv[1] : sby[o];
     const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(say) say[o] = a;
+    if(say) tck_st(&say[o], a);
     if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, Fuy_0[of] + (double)delta, false);
-    else dyo[o] = delta;
+    else tck_st(&dyo[o], delta);
   }
 
   TCK_SECTION_FENCE();
@@ -337,9 +337,9 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     R b = R(0.);
     if(sbz) b = sbv ? sbv[2] : sbz[o];
     const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(saz) saz[o] = a;
+    if(saz) tck_st(&saz[o], a);
     if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, Fuz_0[of] + (double)delta, false);
-    else dzo[o] = delta;
+    else tck_st(&dzo[o], delta);
   }
 
   TCK_SECTION_FENCE();
@@ -386,9 +386,9 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     R b = R(0.);
     if(sbt) b = sbv ? sbv[3] : sbt[o];
     const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
-    if(sat) sat[o] = a;
+    if(sat) tck_st(&sat[o], a);
     if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, Ft_0[of] + (double)delta, false);
-    else dto[o] = delta;
+    else tck_st(&dto[o], delta);
   }
 }
 
