@@ -26,6 +26,37 @@ __device__ __forceinline__ long long tck_idx(const tc_geom_t & g, int i, int j, 
  * images (src/halo.c:18-142 fills j = 0 from ny, j = ny+1 from 1, then the z planes over the
  * full (x, y) extent, so corners are images of images).  With xcopy set the value is also
  * written to the x wall slot next to it (Neumann copy of src/fluid/boundary/{p,psi}.c). */
+typedef double tck_d2v __attribute__((ext_vector_type(2)));
+typedef float tck_f2v __attribute__((ext_vector_type(2)));
+#ifndef TCK_NT_LOAD
+#define TCK_NT_LOAD 1      /* operands read once are loaded non-temporally */
+#endif
+/* load of a streamed operand (read once by this kernel) */
+template <typename T>
+__device__ __forceinline__ T tck_ld(const T * p){
+#if TCK_NT_LOAD
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ double2 tck_ld(const double2 * p){
+#if TCK_NT_LOAD
+  const tck_d2v w = __builtin_nontemporal_load((const tck_d2v *)p);
+  return make_double2(w.x, w.y);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ float2 tck_ld(const float2 * p){
+#if TCK_NT_LOAD
+  const tck_f2v w = __builtin_nontemporal_load((const tck_f2v *)p);
+  return make_float2(w.x, w.y);
+#else
+  return *p;
+#endif
+}
+
 /* store of a streamed result (read again only by a later kernel) */
 template <typename T>
 __device__ __forceinline__ void tck_st(T * p, T v){
@@ -35,8 +66,6 @@ __device__ __forceinline__ void tck_st(T * p, T v){
   *p = v;
 #endif
 }
-typedef double tck_d2v __attribute__((ext_vector_type(2)));
-typedef float tck_f2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void tck_st(double2 * p, double2 v){
 #if TCK_NT_STORE
   const tck_d2v w = {v.x, v.y};

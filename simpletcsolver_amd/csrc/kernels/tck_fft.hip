@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
     const int t = e % tz;
     const int kz = e / tz;
     const int ii = i0 + t;
-    sm[t * A.ld + fpad(kz)] = ii < A.nx ? base[ii + kstride * kz] : mk2<T>(T(0.), T(0.));
+    sm[t * A.ld + fpad(kz)] = ii < A.nx ? tck_ld(&base[ii + kstride * kz]) : mk2<T>(T(0.), T(0.));
   }
   __syncthreads();
   lds_fft<T>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(FFT_BOUND(T)) tck_zfft_p(const ZArgs A){
     const int t_ = (e) % tz, kz_ = (e) / tz; \
     const int ii_ = (int)((tl) % ntx) * tz + t_; \
     const V * b_ = (const V *)A.S + (long long)A.nx * ((tl) / ntx); \
-    out = ((e) < nel && ii_ < A.nx) ? b_[ii_ + kstride * kz_] : mk2<T>(T(0.), T(0.)); }
+    out = ((e) < nel && ii_ < A.nx) ? tck_ld(&b_[ii_ + kstride * kz_]) : mk2<T>(T(0.), T(0.)); }
   if(tile < ntiles){
     #pragma unroll
     for(int q = 0; q < PF; q++){ ZF_ELEM(tile, threadIdx.x + q * nt, pre[q]); }
@@ -414,13 +414,13 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     if(i <= g.nx){
       const long long o = (long long)(i - 1) + rowbase;
       if(kk == 0){
-        V x0 = S[o];
-        V xm = S[o + (long long)g.nx * M];
+        V x0 = tck_ld(&S[o]);
+        V xm = tck_ld(&S[o + (long long)g.nx * M]);
         x0.y = T(0.);
         xm.y = T(0.);
         v = c2r_pre(x0, xm, twN[0]);
       }else{
-        v = S[o + (long long)g.nx * kk];
+        v = tck_ld(&S[o + (long long)g.nx * kk]);
       }
     }
     sm[t * A.ld + fpad(kk)] = v;

@@ -378,8 +378,8 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
       const long long b_ = tck_idx(g, A.i0, 1 + (int)(l_ % g.ny), 1 + (int)(l_ / g.ny)); \
       _Pragma("unroll") for(int e = 0; e < SEG; e++){ \
         const int idx = lane + e * TCK_WAVE; \
-        pd[e] = idx < n ? dd[b_ + idx] : D(0.); \
-        if(ucur) pu[e] = idx < n ? ucur[b_ + idx] : 0.; \
+        pd[e] = idx < n ? tck_ld(&dd[b_ + idx]) : D(0.); \
+        if(ucur) pu[e] = idx < n ? tck_ld(&ucur[b_ + idx]) : 0.; \
       } \
     } }
   XS_FETCH((long long)blockIdx.x * wpb + w);
@@ -529,7 +529,7 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   Q * smX = smA + 2 * S * TI;     /* [3][TI]: q0[0], q0[n-2], q[n-1] */
   Q q[MAXM];
   #pragma unroll
-  for(int e = 0; e < MAXM; e++) q[e] = (valid && e < cnt) ? Q(d[base + step * (row0 + e)]) : Q(0.);
+  for(int e = 0; e < MAXM; e++) q[e] = (valid && e < cnt) ? Q(tck_ld(&d[base + step * (row0 + e)])) : Q(0.);
   if(valid && sg == S - 1) smX[2 * TI + ti] = Q(d[base + step * m]);
   Q t[PS_PRE];
   /* forward: q0 = q0 / c0, q_i = r_i (q_i - l q_{i-1}) */
