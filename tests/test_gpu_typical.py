@@ -22,7 +22,7 @@ with open(os.path.join(ROOT, "tests", "golden", "typical_case.json")) as fh:
 LENGTHS = tuple(CASE["lengths"])
 
 
-def _run(tmp, seed, timemax):
+def _run(tmp, seed, timemax, precision=None):
     from simpletcsolver_amd import icdir
     icdir.write(os.path.join(tmp, "ic"), CASE["is_curved"], LENGTHS, tuple(CASE["glsizes"]), seed=seed)
     for sub in ("log", "save", "stat"):
@@ -32,6 +32,8 @@ def _run(tmp, seed, timemax):
                 "save_after": "0", "stat_rate": "0.1", "stat_after": "1000",
                 "coef_dt_adv": repr(CASE["coef_dt_adv"]), "coef_dt_dif": repr(CASE["coef_dt_dif"]),
                 "Re": repr(CASE["Re"]), "Pr": repr(CASE["Pr"])})
+    if precision:
+        env["TC_PRECISION"] = precision
     print(f"tcsolver typical case, seed {seed}, timemax {timemax}", flush=True)
     r = subprocess.run([EXE, "ic"], cwd=tmp, env=env, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -58,3 +60,18 @@ def test_typical_case_nusselt(need_gpu, tmp_path, seed):
     assert abs(nu_i - nu_d) <= 1e-5 * nu_i
     assert abs(inj[late, 2].mean() - dis[late, 2].mean()) <= 1e-5 * abs(inj[late, 2].mean())
     assert div[:, 1].max() <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["mixed", "mixed_refined"])
+def test_typical_case_mixed_precision(need_gpu, tmp_path, precision):
+    """the same run through the reference API with TC_PRECISION set: the steady Nusselt number
+    within 1e-4 of the FP64 run's 1.13453; the refined form keeps FP64-level divergence"""
+    from simpletcsolver_amd import icdir
+    inj, dis, div = _run(str(tmp_path), 0, 200.0, precision)
+    f = icdir.laminar_torque_factor(CASE["Re"], LENGTHS)
+    late = inj[:, 0] > 150.0
+    nu = inj[late, 1].mean() / f
+    print(f"{precision}: Nu {nu:.5f}, max div {div[:, 1].max():.1e}")
+    assert abs(nu - 1.13453) <= 1e-4 * 1.13453
+    assert div[:, 1].max() <= (1e-12 if precision == "mixed_refined" else 1e-6)
+
