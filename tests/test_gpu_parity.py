@@ -56,7 +56,7 @@ def test_predict_substep_matches_oracle(need_gpu, implicit, window):
         _cmp(o, s, VEL + ("t",), case, tol)
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(70, 24, 12), (33, 8, 6)])
+@pytest.mark.parametrize("nx,ny,nz", [(70, 24, 12), (33, 8, 6), (40, 36, 12)])
 def test_window_predictor_tiles_match_oracle(need_gpu, nx, ny, nz):
     """the LDS-window predictor over several 32 x 8 tiles, partial in i and j: bit for bit"""
     case = make_case(nx=nx, ny=ny, nz=nz, implicit=(0, 0, 0), t_amp=1e-2)
