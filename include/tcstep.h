@@ -160,7 +160,9 @@ int tc_cnn_train_step(tc_cnn_t * c, const float * x, const float * y, float lr, 
 /* couple a batch-1 network of the solver's grid (null: detach): at the start of every step
  * the inputs are formed from the current fields ((q - in_mean) / in_std), the network runs on
  * the solver's stream, and out_scale times its outputs are added to the explicit right-hand
- * sides (ux at faces as the mean of the two neighbouring centres) in every substep */
+ * sides (ux at faces as the mean of the two neighbouring centres) in every substep.  A
+ * network is coupled to one solver at a time (its activations are that solver's scratch);
+ * destroying either side uncouples them */
 int tc_closure_attach(tc_solver_t * s, tc_cnn_t * c, const float in_mean[5], const float in_std[5],
     const float out_scale[4]);
 

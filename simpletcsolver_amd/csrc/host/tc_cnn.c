@@ -50,6 +50,8 @@ static const int g_co[3] = {CNN_C, CNN_C, CNN_CO};
 
 struct tc_cnn_s {
   int nx, ny, nz, nb;
+  tc_solver_t * owner;          /* the solver it is coupled to (its activations are that
+                                 * solver's scratch), or null */
   long long nv;
   hipStream_t stream;
   int device;
@@ -93,6 +95,7 @@ static unsigned short bf16(float x){
 
 int tc_cnn_destroy(tc_cnn_t * c){
   if(!c) return 0;
+  if(c->owner) c->owner->closure = NULL;
   hipSetDevice(c->device);
   if(c->stream) hipStreamSynchronize(c->stream);
   for(int l = 0; l < 3; l++){
@@ -291,9 +294,12 @@ int tc_cnn_train_step(tc_cnn_t * c, const float * x, const float * y, float lr, 
 int tc_closure_attach(tc_solver_t * s, tc_cnn_t * c, const float in_mean[5], const float in_std[5],
     const float out_scale[4]){
   if(!c){
+    if(s->closure) s->closure->owner = NULL;
     s->closure = NULL;
     return 0;
   }
+  if(c->owner && c->owner != s)
+    return tc_fail("tc_closure_attach: the network is coupled to another solver (one network per solver)");
   if(c->nx != s->p.nx || c->ny != s->p.ny || c->nz != s->p.nz || c->nb != 1)
     return tc_fail("tc_closure_attach: network grid %d x %d x %d (batch %d) does not match the solver's",
         c->nx, c->ny, c->nz, c->nb);
@@ -313,7 +319,9 @@ int tc_closure_attach(tc_solver_t * s, tc_cnn_t * c, const float in_mean[5], con
     s->cnn_inv_std[q] = in_std[q] > 0.f ? 1.f / in_std[q] : 1.f;
   }
   for(int q = 0; q < 4; q++) s->cnn_scale[q] = out_scale[q];
+  if(s->closure && s->closure != c) s->closure->owner = NULL;
   s->closure = c;
+  c->owner = s;
   return 0;
 }
 
@@ -329,4 +337,10 @@ int tc_closure_eval(tc_solver_t * s){
   KOK(tck_cnn_launch_scatter(&s->g, c->out, s->cnn_scale, (float *)s->d_corr[0], (float *)s->d_corr[1],
       (float *)s->d_corr[2], s->p.scalar ? (float *)s->d_corr[3] : NULL, s->stream));
   return 0;
+}
+
+/* a solver is destroyed: its network is free to be coupled elsewhere */
+void tc_closure_release(tc_solver_t * s){
+  if(s->closure) s->closure->owner = NULL;
+  s->closure = NULL;
 }

@@ -113,6 +113,7 @@ int tc_enqueue_substep(tc_solver_t * s, int r);
 void tc_advance_host_state(tc_solver_t * s);
 /* make the solver enqueue on a stream it does not own */
 int tc_share_stream(tc_solver_t * s, hipStream_t stream);
+void tc_closure_release(tc_solver_t * s);
 /* the attached network on the current fields -> corrections (tc_cnn.c) */
 int tc_closure_eval(tc_solver_t * s);
 

@@ -446,6 +446,7 @@ int tc_destroy(tc_solver_t * s){
   if(!s) return 0;
   hipSetDevice(s->p.device);
   if(s->stream) hipStreamSynchronize(s->stream);
+  tc_closure_release(s);
   for(int n = 0; n < s->nalloc; n++) hipFree(s->allocs[n]);
   for(int n = 0; n < TC_NSTAGE; n++){
     for(int e = 0; e < TC_MAXEV; e++){
