@@ -482,7 +482,14 @@ int tck_fft_supported(int n){
 
 /* Tile of TB transforms of length M per block: TB = 16 (one 128 B line of doubles, or two
  * lines of complex values, per row access) when the LDS allows it, fewer for very long
- * transforms; threads = TB * M / FFT_EPT. */
+ * transforms; threads = TB * M / FFT_EPT, or TB * M / (FFT_EPT / 2) for tiles so large that
+ * the LDS holds too few blocks per CU (measured on the FP64 transforms of 512x1024x1024:
+ * 7.2 -> 5.6 ms for the divergence + theta transform, 4.6 -> 3.7 ms for the inverse theta
+ * transform; the FP32 tiles, already at 16 waves per CU, were slower with twice the
+ * threads). */
+#ifndef TCK_FFT_HALFEPT
+#define TCK_FFT_HALFEPT 1    /* LDS-bound tiles: FFT_EPT / 2 elements per thread */
+#endif
 #ifndef TCK_FFT_YSHARE
 #define TCK_FFT_YSHARE 1     /* theta transforms: blocks per CU the tiles are sized for */
 #endif
@@ -534,7 +541,15 @@ static int fft_tile(int M, int cap, size_t csize, int * threads, int share, long
   while(b * 2 <= cap && b * 2 * M <= FFT_EPT * 64) b *= 2;   /* short transforms: wider tiles */
   while(b > 1 && lines / b < 2LL * fft_ncu()) b /= 2;
   if(b > cap) b = cap < 1 ? 1 : cap;
-  int t = (b * M + FFT_EPT - 1) / FFT_EPT;
+  /* elements per thread: FFT_EPT, or half that where the LDS holds so few blocks per CU that
+   * FFT_EPT would leave fewer than 16 waves on it (FP64 tiles of long transforms: one block
+   * per CU).  The device passes loop over their butterflies, so any count up to FFT_EPT
+   * computes the same bits. */
+  int ept = FFT_EPT;
+  const long long per = (long long)(lds_limit() / ((size_t)b * lb));
+  const int w16 = ((b * M + FFT_EPT - 1) / FFT_EPT + 63) / 64;
+  if(TCK_FFT_HALFEPT && per * w16 < 16 && (b * M + FFT_EPT / 2 - 1) / (FFT_EPT / 2) <= FFT_MAXTHREADS) ept = FFT_EPT / 2;
+  int t = (b * M + ept - 1) / ept;
   t = (t + 63) / 64 * 64;
   if(t < 64) t = 64;
   *threads = t;
