@@ -113,11 +113,11 @@ template <typename T> struct dftr<8, T> {
  * the row-wise fills spread over all banks */
 __host__ __device__ __forceinline__ int fpad(int n){ return n + (n >> 3); }
 
-template <int R, typename T>
+template <int R, typename T, int EPT = FFT_EPT>
 __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, int ld, int nb, int Ns,
     const typename cvec<T>::type * __restrict__ tw, T s){
   typedef typename cvec<T>::type V;
-  constexpr int BPT = (FFT_EPT + R - 1) / R;
+  constexpr int BPT = (EPT + R - 1) / R;
   const int nt = blockDim.x;
   const int tid = threadIdx.x;
   const int per = M / R;
@@ -162,17 +162,18 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
   __syncthreads();
 }
 
-/* nb transforms of length P.M stored at buf + b * ld; tw[t] = exp(-2 pi i t / M) */
-template <typename T>
+/* nb transforms of length P.M stored at buf + b * ld; tw[t] = exp(-2 pi i t / M); each pass
+ * keeps at most EPT elements per thread in registers, so blockDim.x * EPT >= nb * P.M */
+template <typename T, int EPT = FFT_EPT>
 __device__ void lds_fft(typename cvec<T>::type * buf, int ld, int nb, const FftPlan & P,
     const typename cvec<T>::type * __restrict__ tw, T s){
   int Ns = 1;
   for(int st = 0; st < P.nst; st++){
     const int R = P.rad[st];
-    if(R == 8) fft_stage<8, T>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(R == 4) fft_stage<4, T>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(R == 3) fft_stage<3, T>(buf, P.M, ld, nb, Ns, tw, s);
-    else fft_stage<2, T>(buf, P.M, ld, nb, Ns, tw, s);
+    if(R == 8) fft_stage<8, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(R == 4) fft_stage<4, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(R == 3) fft_stage<3, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
+    else fft_stage<2, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
     Ns *= R;
   }
 }
@@ -303,12 +304,13 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
 
 /* Persistent form: each block loops over tiles and issues the loads of the first PF of its
  * elements of the next tile before the current tile's transform, so that part of the HBM
- * traffic overlaps the LDS passes (the remaining elements are loaded when the tile starts). */
+ * traffic overlaps the LDS passes (the remaining elements are loaded when the tile starts).
+ * Compiled for FFT_EPT / 2 elements per thread per pass, the launches whose tiles take
+ * FFT_EPT / 2 (FP64 tiles of long transforms, 1024 threads per block). */
 #ifndef TCK_ZFFT_PF
-#define TCK_ZFFT_PF 8            /* elements per thread loaded one tile ahead (FP64 only: the
-                                  * FP32 form, at 1024 threads per block, would spill) */
+#define TCK_ZFFT_PF 2            /* elements per thread loaded one tile ahead (FP64 only) */
 #endif
-#define FFT_BOUND(T) (sizeof(T) == 8 ? 512 : FFT_MAXTHREADS)
+#define ZP_EPT (FFT_EPT / 2)
 
 __device__ __forceinline__ void fft_lds_barrier(){
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -317,7 +319,7 @@ __device__ __forceinline__ void fft_lds_barrier(){
 }
 
 template <typename T, int PF>
-__global__ void __launch_bounds__(FFT_BOUND(T)) tck_zfft_p(const ZArgs A){
+__global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
   V * sm = (V *)fft_smem;
@@ -352,7 +354,7 @@ __global__ void __launch_bounds__(FFT_BOUND(T)) tck_zfft_p(const ZArgs A){
       #pragma unroll
       for(int q = 0; q < PF; q++){ ZF_ELEM(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
     }
-    lds_fft<T>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
+    lds_fft<T, ZP_EPT>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
     const int i0 = (int)(tile % ntx) * tz;
     V * base = (V *)A.S + (long long)A.nx * (tile / ntx);
     for(int e = threadIdx.x; e < nel; e += nt){
@@ -618,7 +620,7 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
   const size_t lds = cs * (size_t)A.tz * A.ld;
   const long long ntiles = (long long)((nx + A.tz - 1) / A.tz) * nky;
 #if TCK_ZFFT_PF > 0
-  if(!mixed && nt <= FFT_BOUND(double)){
+  if(!mixed && (long long)nt * ZP_EPT >= (long long)A.tz * nz){
     const void * fn = (const void *)tck_zfft_p<double, TCK_ZFFT_PF>;
     if(set_lds(fn, lds)) return (int)hipErrorInvalidValue;
     /* resident blocks per CU, queried once per (threads, LDS) so that launches recorded into

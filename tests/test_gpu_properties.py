@@ -46,6 +46,40 @@ def test_device_manufactured_potential(need_gpu, curved, shape):
     s.close()
 
 
+def test_device_manufactured_potential_long_lines(need_gpu):
+    """the same at 128 x 256 x 1024: the axial transforms (length 1024) take the persistent
+    kernel with tiles of 8 lines and 1024 threads, the theta transforms twice the threads of a
+    short tile.  The radial metrics come from the oracle on 4 planes (x and theta metrics do
+    not depend on nz).  Tolerance: the smallest Poisson eigenvalues fall as 1/nz^2, so the
+    round-off of the right-hand side reaches psi amplified by up to nz^2 eps."""
+    from simpletcsolver_amd import Solver
+    nx, ny, nz = 128, 256, 1024
+    small = make_case(nx=nx, ny=ny, nz=4, Re=100.0, amp=0.0, uy_walls=(0.0, 0.0))
+    o = oracle_for(small)
+    hxxf, hyxc = o.array("hxxf").copy(), o.array("hyxc").copy()
+    o.close()
+    hz = small["kw"]["lengths"][2] / nz
+    rng = np.random.default_rng(29)
+    psi = rng.standard_normal((nz, ny, nx))
+    rk, dt = 1, 0.13
+    g = orc.rkcoefs()[rk, 2] * dt
+    ux = np.zeros((nz, ny, nx + 1))
+    ux[:, :, 1:nx] = g * (psi[:, :, 1:] - psi[:, :, :-1]) / hxxf[1:nx]
+    uy = np.zeros((nz, ny, nx + 2))
+    uy[:, :, 1:nx + 1] = g * (psi - np.roll(psi, 1, axis=1)) / hyxc[1:nx + 1]
+    uz = np.zeros((nz, ny, nx + 2))
+    uz[:, :, 1:nx + 1] = g * (psi - np.roll(psi, 1, axis=0)) / hz
+    s = Solver(nx, ny, nz, small["xf"], small["xc"], **small["kw"])
+    s.load_fields({"ux": ux, "uy": uy, "uz": uz})
+    s.potential(rk, dt)
+    got = s.get_field("psi")[:, :, 1:-1]
+    s.close()
+    d = got - psi
+    err = np.abs(d - d.mean()).max() / np.abs(psi).max()
+    print(f"manufactured psi {(nx, ny, nz)}: {err:.3e}")
+    assert err <= 1e-9
+
+
 @pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 1, 1)])
 def test_device_step_is_solenoidal(need_gpu, implicit):
     case = make_case(nx=24, ny=16, nz=12, Re=300.0, implicit=implicit, amp=0.3)
