@@ -25,6 +25,7 @@
 
 #define FFT_MAXST 16
 #define FFT_EPT 16   /* complex elements per thread per pass */
+#define ZP_EPT (FFT_EPT / 2)   /* the persistent kernels, which run LDS-bound FP64 tiles */
 #define FFT_MAXTHREADS 1024
 
 /* complex pair of the transform's arithmetic type T (double, or float in the mixed step) */
@@ -189,6 +190,14 @@ __device__ __forceinline__ long long fft_tile_of(long long ntiles){
   return t;
 }
 
+/* block barrier that orders the LDS accesses only: global loads issued before it stay in
+ * flight and pending global stores are not waited for */
+__device__ __forceinline__ void fft_lds_barrier(){
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 /* ------------------------------------------------------------------------------------- */
 struct YFwdArgs {
   tc_geom_t g;
@@ -263,6 +272,104 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
   }
 }
 
+/* Persistent form (FP64 tiles of long transforms, compiled for FFT_EPT / 2 elements per
+ * thread per pass): each block loops over tiles and loads the velocities of the first PF of
+ * its divergence elements of the next tile before the current tile's passes. */
+#ifndef TCK_DIVY_PF
+#define TCK_DIVY_PF 1
+#endif
+
+template <typename T, int PF>
+__global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs A){
+  typedef typename cvec<T>::type V;
+  extern __shared__ __align__(16) unsigned char fft_smem[];
+  V * sm = (V *)fft_smem;
+  const tc_geom_t & g = A.g;
+  const tc_coef_t & c = A.c;
+  const int ti = A.ti;
+  const int M = A.P.M;
+  const int nt = blockDim.x;
+  const int nel = ti * g.ny;
+  const int ntx = (g.nx + ti - 1) / ti;
+  const long long ntiles = (long long)ntx * g.nz;
+  const double dt = *A.dt;
+  const double gamma = c_fft_rka[A.rkstep] + c_fft_rkb[A.rkstep];
+  const double norm = 1. * g.ny * g.nz;
+  const double prefactor = 1. / (gamma * dt) / norm;
+  const long long sy = g.px;
+  const long long sz = g.plane;
+  const V * twM = (const V *)A.twM;
+  const V * twN = (const V *)A.twN;
+  V * S = (V *)A.S;
+  double pre[PF][6];
+  /* the six velocities of element e of tile tl (zero outside the field) */
+  #define DV_LOAD(tl, e, P) { \
+    const int t_ = (e) % ti, j_ = (e) / ti; \
+    const int i_ = 1 + (int)((tl) % ntx) * ti + t_; \
+    if((e) < nel && i_ <= g.nx){ \
+      const long long o_ = tck_idx(g, i_, j_ + 1, 1 + (int)((tl) / ntx)); \
+      P[0] = A.ux[o_]; P[1] = A.ux[o_ + 1]; P[2] = A.uy[o_]; P[3] = A.uy[o_ + sy]; \
+      P[4] = A.uz[o_]; P[5] = A.uz[o_ + sz]; \
+    }else{ P[0] = P[1] = P[2] = P[3] = P[4] = P[5] = 0.; } }
+  long long tile = blockIdx.x;
+  if(tile < ntiles){
+    #pragma unroll
+    for(int q = 0; q < PF; q++){ DV_LOAD(tile, threadIdx.x + q * nt, pre[q]); }
+  }
+  for(; tile < ntiles; tile += gridDim.x){
+    const int i0 = 1 + (int)(tile % ntx) * ti;
+    const int k = 1 + (int)(tile / ntx);
+    /* divergence (assign_input :404-423, same operand order as tck_div_yfft) packed as
+     * z_m = x_2m + i x_2m+1 */
+    auto put = [&](const int e, const double * P){
+      const int t = e % ti;
+      const int j0 = e / ti;
+      const int i = i0 + t;
+      double val = 0.;
+      if(i <= g.nx){
+        const double div = c.ijdxc[i] * (
+            - c.jhxf[i] * P[0] + c.jhxf[i + 1] * P[1]
+            - c.jhyc[i] * P[2] + c.jhyc[i] * P[3]
+            - c.jhzc[i] * P[4] + c.jhzc[i] * P[5]);
+        val = prefactor * div;
+      }
+      ((T *)(sm + t * A.ld))[2 * fpad(j0 >> 1) + (j0 & 1)] = T(val);
+    };
+    #pragma unroll
+    for(int q = 0; q < PF; q++){
+      const int e = threadIdx.x + q * nt;
+      if(e < nel) put(e, pre[q]);
+    }
+    for(int e = threadIdx.x + PF * nt; e < nel; e += nt){
+      double P[6];
+      DV_LOAD(tile, e, P);
+      put(e, P);
+    }
+    fft_lds_barrier();
+    if(tile + gridDim.x < ntiles){
+      #pragma unroll
+      for(int q = 0; q < PF; q++){ DV_LOAD(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
+    }
+    lds_fft<T, ZP_EPT>(sm, A.ld, ti, A.P, twM, T(-1.));
+    for(int e = threadIdx.x; e < ti * (M + 1); e += nt){
+      const int t = e % ti;
+      const int kk = e / ti;
+      const int i = i0 + t;
+      if(i > g.nx) continue;
+      const V zk = sm[t * A.ld + fpad(kk == M ? 0 : kk)];
+      const V zc = cconj(sm[t * A.ld + fpad(kk == 0 ? 0 : M - kk)]);
+      const V w = twN[kk];
+      const V ev = mk2<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
+      const V od = mk2<T>(T(0.5) * (zk.x - zc.x), T(0.5) * (zk.y - zc.y));
+      const V wo = cmul(w, od);
+      const V X = mk2<T>(ev.x + wo.y, ev.y - wo.x);
+      tck_st(&S[(long long)(i - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))], X);
+    }
+    fft_lds_barrier();
+  }
+  #undef DV_LOAD
+}
+
 /* ------------------------------------------------------------------------------------- */
 struct ZArgs {
   int nx, nky, nz, tz, ld;
@@ -310,13 +417,6 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
 #ifndef TCK_ZFFT_PF
 #define TCK_ZFFT_PF 2            /* elements per thread loaded one tile ahead (FP64 only) */
 #endif
-#define ZP_EPT (FFT_EPT / 2)
-
-__device__ __forceinline__ void fft_lds_barrier(){
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 template <typename T, int PF>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
@@ -454,6 +554,101 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     const T v = ((const T *)(sm + t * A.ld))[2 * fpad(jj >> 1) + (jj & 1)];
     tck_store_halo(psi, g, i, jj + 1, k, v, true);
   }
+}
+
+/* Persistent form of tck_yifft_psi (FP64 tiles of long transforms, compiled for ZP_EPT
+ * elements per thread per pass): each block loops over tiles and issues the spectrum loads of
+ * the first PF of its elements of the next tile before the current tile's passes. */
+#ifndef TCK_YIFFT_PF
+#define TCK_YIFFT_PF 2
+#endif
+
+template <typename T, int PF>
+__global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs A){
+  typedef typename cvec<T>::type V;
+  extern __shared__ __align__(16) unsigned char fft_smem[];
+  V * sm = (V *)fft_smem;
+  const tc_geom_t & g = A.g;
+  const int ti = A.ti;
+  const int M = A.P.M;
+  const int nt = blockDim.x;
+  const int nel = ti * M;
+  const int ntx = (g.nx + ti - 1) / ti;
+  const long long ntiles = (long long)ntx * g.nz;
+  const V * S = (const V *)A.S;
+  const V * twN = (const V *)A.twN;
+  T * psi = (T *)A.psi;
+  V pre[PF];
+  long long tile = blockIdx.x;
+  /* raw spectrum value X_kk of element e of tile tl */
+  #define YI_ELEM(tl, e, out) { \
+    const int t_ = (e) % ti, kk_ = (e) / ti; \
+    const int i_ = 1 + (int)((tl) % ntx) * ti + t_; \
+    const long long o_ = (long long)(i_ - 1) + (long long)g.nx * ((long long)A.nky * ((tl) / ntx) + kk_); \
+    out = ((e) < nel && i_ <= g.nx) ? tck_ld(&S[o_]) : mk2<T>(T(0.), T(0.)); }
+  if(tile < ntiles){
+    #pragma unroll
+    for(int q = 0; q < PF; q++){ YI_ELEM(tile, threadIdx.x + q * nt, pre[q]); }
+  }
+  for(; tile < ntiles; tile += gridDim.x){
+    const int i0 = 1 + (int)(tile % ntx) * ti;
+    const int k = 1 + (int)(tile / ntx);
+    const long long rowbase = (long long)g.nx * (long long)A.nky * (k - 1);
+    /* X_1..X_{M-1} as loaded; Z'_0 from X_0 and X_M */
+    auto put = [&](const int e, V v){
+      const int t = e % ti;
+      const int kk = e / ti;
+      const int i = i0 + t;
+      if(kk == 0 && i <= g.nx){
+        V xm = tck_ld(&S[(long long)(i - 1) + rowbase + (long long)g.nx * M]);
+        v.y = T(0.);
+        xm.y = T(0.);
+        v = c2r_pre(v, xm, twN[0]);
+      }
+      sm[t * A.ld + fpad(kk)] = v;
+    };
+    #pragma unroll
+    for(int q = 0; q < PF; q++){
+      const int e = threadIdx.x + q * nt;
+      if(e < nel) put(e, pre[q]);
+    }
+    for(int e = threadIdx.x + PF * nt; e < nel; e += nt){
+      V v;
+      YI_ELEM(tile, e, v);
+      put(e, v);
+    }
+    fft_lds_barrier();
+    if(tile + gridDim.x < ntiles){
+      #pragma unroll
+      for(int q = 0; q < PF; q++){ YI_ELEM(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
+    }
+    const int half = M / 2;
+    for(int e = threadIdx.x; e < ti * half; e += nt){
+      const int t = e % ti;
+      const int k1 = 1 + e / ti;
+      const int k2 = M - k1;
+      const V xa = sm[t * A.ld + fpad(k1)];
+      const V xb = sm[t * A.ld + fpad(k2)];
+      const V z1 = c2r_pre(xa, xb, twN[k1]);
+      if(k2 != k1){
+        const V z2 = c2r_pre(xb, xa, twN[k2]);
+        sm[t * A.ld + fpad(k2)] = z2;
+      }
+      sm[t * A.ld + fpad(k1)] = z1;
+    }
+    fft_lds_barrier();
+    lds_fft<T, ZP_EPT>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
+    for(int e = threadIdx.x; e < ti * g.ny; e += nt){
+      const int t = e % ti;
+      const int jj = e / ti;
+      const int i = i0 + t;
+      if(i > g.nx) continue;
+      const T v = ((const T *)(sm + t * A.ld))[2 * fpad(jj >> 1) + (jj & 1)];
+      tck_store_halo(psi, g, i, jj + 1, k, v, true);
+    }
+    fft_lds_barrier();
+  }
+  #undef YI_ELEM
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -596,7 +791,28 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   A.twM = twM; A.twN = twN;
   A.ld = fft_ld(A.P.M);
   const size_t lds = cs * (size_t)A.ti * A.ld;
-  const dim3 grd((unsigned)((long long)((g->nx + A.ti - 1) / A.ti) * g->nz));
+  const long long ntiles = (long long)((g->nx + A.ti - 1) / A.ti) * g->nz;
+#if TCK_DIVY_PF > 0
+  if(!mixed && (long long)nt * ZP_EPT >= (long long)A.ti * A.P.M){
+    const void * fn = (const void *)tck_div_yfft_p<double, TCK_DIVY_PF>;
+    if(set_lds(fn, lds)) return (int)hipErrorInvalidValue;
+    static int c_nt = 0, c_per = 0;
+    static size_t c_lds = 0;
+    int per = 0;
+    if(c_nt == nt && c_lds == lds) per = c_per;
+    else{
+      if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
+      c_nt = nt; c_lds = lds; c_per = per;
+    }
+    const long long nb = (long long)per * fft_ncu();
+    if(ntiles >= 4 * nb){
+      hipLaunchKernelGGL((tck_div_yfft_p<double, TCK_DIVY_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
+      TCK_CHECK_LAUNCH();
+      return 0;
+    }
+  }
+#endif
+  const dim3 grd((unsigned)ntiles);
   if(mixed){
     if(set_lds((const void *)tck_div_yfft<float>, lds)) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(tck_div_yfft<float>, grd, dim3(nt), lds, stream, A);
@@ -665,7 +881,28 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
   A.twM = twM; A.twN = twN;
   A.ld = fft_ld(A.P.M);
   const size_t lds = cs * (size_t)A.ti * A.ld;
-  const dim3 grd((unsigned)((long long)((g->nx + A.ti - 1) / A.ti) * g->nz));
+  const long long ntiles = (long long)((g->nx + A.ti - 1) / A.ti) * g->nz;
+#if TCK_YIFFT_PF > 0
+  if(!mixed && (long long)nt * ZP_EPT >= (long long)A.ti * A.P.M){
+    const void * fn = (const void *)tck_yifft_psi_p<double, TCK_YIFFT_PF>;
+    if(set_lds(fn, lds)) return (int)hipErrorInvalidValue;
+    static int c_nt = 0, c_per = 0;
+    static size_t c_lds = 0;
+    int per = 0;
+    if(c_nt == nt && c_lds == lds) per = c_per;
+    else{
+      if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
+      c_nt = nt; c_lds = lds; c_per = per;
+    }
+    const long long nb = (long long)per * fft_ncu();
+    if(ntiles >= 4 * nb){
+      hipLaunchKernelGGL((tck_yifft_psi_p<double, TCK_YIFFT_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
+      TCK_CHECK_LAUNCH();
+      return 0;
+    }
+  }
+#endif
+  const dim3 grd((unsigned)ntiles);
   if(mixed){
     if(set_lds((const void *)tck_yifft_psi<float>, lds)) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(tck_yifft_psi<float>, grd, dim3(nt), lds, stream, A);
