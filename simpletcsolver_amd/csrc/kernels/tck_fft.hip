@@ -25,7 +25,6 @@
 
 #define FFT_MAXST 16
 #define FFT_EPT 16   /* complex elements per thread per pass */
-#define ZP_EPT (FFT_EPT / 2)   /* the persistent kernels, which run LDS-bound FP64 tiles */
 #define FFT_MAXTHREADS 1024
 
 /* complex pair of the transform's arithmetic type T (double, or float in the mixed step) */
@@ -272,14 +271,14 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
   }
 }
 
-/* Persistent form (FP64 tiles of long transforms, compiled for FFT_EPT / 2 elements per
- * thread per pass): each block loops over tiles and loads the velocities of the first PF of
- * its divergence elements of the next tile before the current tile's passes. */
+/* Persistent form (compiled for EPTK elements per thread per pass): each block loops over
+ * tiles and loads the velocities of the first PF of its divergence elements of the next tile
+ * before the current tile's passes. */
 #ifndef TCK_DIVY_PF
 #define TCK_DIVY_PF 1
 #endif
 
-template <typename T, int PF>
+template <typename T, int PF, int EPTK>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -350,7 +349,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
       #pragma unroll
       for(int q = 0; q < PF; q++){ DV_LOAD(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
     }
-    lds_fft<T, ZP_EPT>(sm, A.ld, ti, A.P, twM, T(-1.));
+    lds_fft<T, EPTK>(sm, A.ld, ti, A.P, twM, T(-1.));
     for(int e = threadIdx.x; e < ti * (M + 1); e += nt){
       const int t = e % ti;
       const int kk = e / ti;
@@ -412,13 +411,12 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
 /* Persistent form: each block loops over tiles and issues the loads of the first PF of its
  * elements of the next tile before the current tile's transform, so that part of the HBM
  * traffic overlaps the LDS passes (the remaining elements are loaded when the tile starts).
- * Compiled for FFT_EPT / 2 elements per thread per pass, the launches whose tiles take
- * FFT_EPT / 2 (FP64 tiles of long transforms, 1024 threads per block). */
+ * Compiled for EPTK elements per thread per pass. */
 #ifndef TCK_ZFFT_PF
-#define TCK_ZFFT_PF 2            /* elements per thread loaded one tile ahead (FP64 only) */
+#define TCK_ZFFT_PF 2            /* elements per thread loaded one tile ahead */
 #endif
 
-template <typename T, int PF>
+template <typename T, int PF, int EPTK>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -454,7 +452,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
       #pragma unroll
       for(int q = 0; q < PF; q++){ ZF_ELEM(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
     }
-    lds_fft<T, ZP_EPT>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
+    lds_fft<T, EPTK>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
     const int i0 = (int)(tile % ntx) * tz;
     V * base = (V *)A.S + (long long)A.nx * (tile / ntx);
     for(int e = threadIdx.x; e < nel; e += nt){
@@ -556,14 +554,14 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
   }
 }
 
-/* Persistent form of tck_yifft_psi (FP64 tiles of long transforms, compiled for ZP_EPT
- * elements per thread per pass): each block loops over tiles and issues the spectrum loads of
- * the first PF of its elements of the next tile before the current tile's passes. */
+/* Persistent form of tck_yifft_psi (compiled for EPTK elements per thread per pass): each
+ * block loops over tiles and issues the spectrum loads of the first PF of its elements of the
+ * next tile before the current tile's passes. */
 #ifndef TCK_YIFFT_PF
 #define TCK_YIFFT_PF 2
 #endif
 
-template <typename T, int PF>
+template <typename T, int PF, int EPTK>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -637,7 +635,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
       sm[t * A.ld + fpad(k1)] = z1;
     }
     fft_lds_barrier();
-    lds_fft<T, ZP_EPT>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
+    lds_fft<T, EPTK>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
     for(int e = threadIdx.x; e < ti * g.ny; e += nt){
       const int t = e % ti;
       const int jj = e / ti;
@@ -777,6 +775,44 @@ static int set_lds(const void * fn, size_t bytes){
   return 0;
 }
 
+/* Persistent launches, FP64 only (the FP32 forms, at 16 elements per thread, spill with the
+ * prefetch registers): the kernels are compiled for FFT_EPT / 2 elements per thread per pass,
+ * the count LDS-bound FP64 tiles take; a launch qualifies when its threads cover the tile at
+ * that count and it has at least four tiles per resident block.  Returns 1 when launched (the caller checks the launch), 0 when the caller should
+ * launch the one-tile form, < 0 when the LDS limit could not be raised.  Resident blocks per CU are queried once per (kernel, threads, LDS), so launches
+ * recorded into a graph make no query. */
+#define PERSIST_CACHE 16
+static const void * g_pc_fn[PERSIST_CACHE];
+static int g_pc_nt[PERSIST_CACHE], g_pc_per[PERSIST_CACHE];
+static size_t g_pc_lds[PERSIST_CACHE];
+static int g_pc_n = 0;
+
+}   /* extern "C" */
+
+template <typename Args>
+static int launch_persistent(void (*kern)(const Args), int eptk, int maxnt, const Args & A, int nt,
+    size_t lds, long long elems, long long ntiles, hipStream_t stream){
+  if((long long)nt * eptk < elems || nt > maxnt) return 0;
+  const void * fn = (const void *)kern;
+  int per = 0;
+  for(int q = 0; q < g_pc_n; q++) if(g_pc_fn[q] == fn && g_pc_nt[q] == nt && g_pc_lds[q] == lds) per = g_pc_per[q];
+  if(per == 0){
+    if(set_lds(fn, lds)) return -1;
+    if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
+    if(g_pc_n < PERSIST_CACHE){
+      g_pc_fn[g_pc_n] = fn; g_pc_nt[g_pc_n] = nt; g_pc_lds[g_pc_n] = lds; g_pc_per[g_pc_n] = per;
+      g_pc_n++;
+    }
+  }
+  const long long nb = (long long)per * fft_ncu();
+  if(ntiles < 4 * nb) return 0;
+  if(set_lds(fn, lds)) return -1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(nt), lds, stream, A);
+  return 1;
+}
+
+extern "C" {
+
 /* mixed: spectrum, transforms and psi in FP32 (twiddles twM, twN, tw as float2) */
 int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
     const double * uy, const double * uz, const double * dt, int rkstep, void * S,
@@ -793,23 +829,11 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   const size_t lds = cs * (size_t)A.ti * A.ld;
   const long long ntiles = (long long)((g->nx + A.ti - 1) / A.ti) * g->nz;
 #if TCK_DIVY_PF > 0
-  if(!mixed && (long long)nt * ZP_EPT >= (long long)A.ti * A.P.M){
-    const void * fn = (const void *)tck_div_yfft_p<double, TCK_DIVY_PF>;
-    if(set_lds(fn, lds)) return (int)hipErrorInvalidValue;
-    static int c_nt = 0, c_per = 0;
-    static size_t c_lds = 0;
-    int per = 0;
-    if(c_nt == nt && c_lds == lds) per = c_per;
-    else{
-      if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
-      c_nt = nt; c_lds = lds; c_per = per;
-    }
-    const long long nb = (long long)per * fft_ncu();
-    if(ntiles >= 4 * nb){
-      hipLaunchKernelGGL((tck_div_yfft_p<double, TCK_DIVY_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
-      TCK_CHECK_LAUNCH();
-      return 0;
-    }
+  {
+    const long long el = (long long)A.ti * A.P.M;
+    const int r = mixed ? 0 : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, FFT_EPT / 2>, FFT_EPT / 2, FFT_MAXTHREADS, A, nt, lds, el, ntiles, stream);
+    if(r < 0) return (int)hipErrorInvalidValue;
+    if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
 #endif
   const dim3 grd((unsigned)ntiles);
@@ -836,26 +860,11 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
   const size_t lds = cs * (size_t)A.tz * A.ld;
   const long long ntiles = (long long)((nx + A.tz - 1) / A.tz) * nky;
 #if TCK_ZFFT_PF > 0
-  if(!mixed && (long long)nt * ZP_EPT >= (long long)A.tz * nz){
-    const void * fn = (const void *)tck_zfft_p<double, TCK_ZFFT_PF>;
-    if(set_lds(fn, lds)) return (int)hipErrorInvalidValue;
-    /* resident blocks per CU, queried once per (threads, LDS) so that launches recorded into
-     * a graph make no query */
-    static int c_nt = 0, c_per = 0;
-    static size_t c_lds = 0;
-    int per = 0, ncu = fft_ncu();
-    if(c_nt == nt && c_lds == lds) per = c_per;
-    else{
-      if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
-      c_nt = nt; c_lds = lds; c_per = per;
-    }
-    /* persistence pays only when every block has several tiles to loop over */
-    const long long nb = (long long)per * ncu;
-    if(ntiles >= 4 * nb){
-      hipLaunchKernelGGL((tck_zfft_p<double, TCK_ZFFT_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
-      TCK_CHECK_LAUNCH();
-      return 0;
-    }
+  {
+    const long long el = (long long)A.tz * nz;
+    const int r = mixed ? 0 : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, FFT_EPT / 2>, FFT_EPT / 2, FFT_MAXTHREADS, A, nt, lds, el, ntiles, stream);
+    if(r < 0) return (int)hipErrorInvalidValue;
+    if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
 #endif
   const dim3 grd((unsigned)ntiles);
@@ -883,23 +892,11 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
   const size_t lds = cs * (size_t)A.ti * A.ld;
   const long long ntiles = (long long)((g->nx + A.ti - 1) / A.ti) * g->nz;
 #if TCK_YIFFT_PF > 0
-  if(!mixed && (long long)nt * ZP_EPT >= (long long)A.ti * A.P.M){
-    const void * fn = (const void *)tck_yifft_psi_p<double, TCK_YIFFT_PF>;
-    if(set_lds(fn, lds)) return (int)hipErrorInvalidValue;
-    static int c_nt = 0, c_per = 0;
-    static size_t c_lds = 0;
-    int per = 0;
-    if(c_nt == nt && c_lds == lds) per = c_per;
-    else{
-      if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, nt, lds) != hipSuccess || per < 1) per = 1;
-      c_nt = nt; c_lds = lds; c_per = per;
-    }
-    const long long nb = (long long)per * fft_ncu();
-    if(ntiles >= 4 * nb){
-      hipLaunchKernelGGL((tck_yifft_psi_p<double, TCK_YIFFT_PF>), dim3((unsigned)nb), dim3(nt), lds, stream, A);
-      TCK_CHECK_LAUNCH();
-      return 0;
-    }
+  {
+    const long long el = (long long)A.ti * A.P.M;
+    const int r = mixed ? 0 : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, FFT_EPT / 2>, FFT_EPT / 2, FFT_MAXTHREADS, A, nt, lds, el, ntiles, stream);
+    if(r < 0) return (int)hipErrorInvalidValue;
+    if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
 #endif
   const dim3 grd((unsigned)ntiles);
