@@ -13,6 +13,10 @@
  *  tck_zfft       complex axial transforms, forward or backward, in place.
  *  tck_yifft_psi  complex-to-real theta transform fused with extract_output (:427-443): psi
  *                 with its periodic halos and Neumann wall copies (boundary/psi.c:13-45).
+ * Each has a persistent form (suffix _p) for FP64 tiles of long transforms: a block loops
+ * over tiles and issues part of the next tile's loads before the current tile's LDS passes.
+ * Tiles whose LDS leaves few blocks per CU run with half the elements per thread (twice the
+ * threads), so that a CU still holds 16 waves.  Every form computes the same bits.
  * Transforms are unnormalised, as FFTW's; the imaginary parts of the DC and Nyquist bins are
  * ignored by the complex-to-real transform, as FFTW's c2r does with a Hermitian input.
  *
