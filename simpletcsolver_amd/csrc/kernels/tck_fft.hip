@@ -755,11 +755,6 @@ static int fft_tile(int M, int cap, size_t csize, int * threads, int share, long
   return b;
 }
 
-int tck_fft_batch(int M, int cap){
-  int t;
-  return fft_tile(M, cap, sizeof(double2), &t, 1, 1LL << 40);
-}
-
 /* raises a kernel's dynamic LDS limit once per (kernel, size): later launches, including
  * those recorded into a graph, make no attribute call */
 #define FFT_LDS_CACHE 32
