@@ -34,10 +34,12 @@ int tck_launch_factor(const double * dt, const int dirs[3], int nx, int ny, int 
     const double (*fac)[3], const int * i0, const int * n,
     const double * const (*lx)[3], const double * const (*ly)[3],
     double * const (*xt)[3], double * const (*yt)[4], double * const (*zt)[4], hipStream_t stream);
-/* d: the increment (float when mixed); uout: where the final pass (ucur non-null) writes
- * ucur + solution (FP64; may be d itself when not mixed) */
-int tck_launch_xsolve(const tc_geom_t * g, void * d, double * uout, const double * ucur,
-    const double * r, const double * v, const double * l, int i0, int n, int mixed, hipStream_t stream);
+/* the x solves of nf (1..4) fields in one launch; per field: d the increment (float when
+ * mixed), uout where the final pass (ucur non-null) writes ucur + solution (FP64; may be d
+ * itself when not mixed), the factor rows r, v, l and the line i0 .. i0+n-1 */
+int tck_launch_xsolve(const tc_geom_t * g, int nf, void * const * d, double * const * uout,
+    const double * const * ucur, const double * const * r, const double * const * v,
+    const double * const * l, const int * i0, const int * n, int mixed, hipStream_t stream);
 int tck_launch_psolve(const tc_geom_t * g, int dir, void * d, double * uout, const double * ucur,
     const double * r, const double * v, const double * q1, const double * meta,
     long long tstride, int tcol, int i0, int isz, int mixed, hipStream_t stream);

@@ -813,15 +813,26 @@ static int predict(tc_solver_t * s, int rkstep){
   if(any_implicit){
     int last = 0;
     for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
+    /* every field's solves run x, y, z in order; the fields are independent, so the x solves
+     * of all fields go in one launch */
+    if(s->p.implicit[0]){
+      void * xd[4]; double * xo[4]; const double * xc[4];
+      const double * xr[4]; const double * xv[4]; const double * xl[4];
+      int xi0[4], xn[4];
+      for(int f = 0; f < nf; f++){
+        tc_ls_t * L = &s->ls[f][rkstep];
+        xd[f] = dlt[f]; xo[f] = nxt[f]; xc[f] = last == 0 ? cur[f] : NULL;
+        xr[f] = L->xr; xv[f] = L->xv; xl[f] = L->xl;
+        xi0[f] = f == TC_FUX ? 2 : 1;
+        xn[f] = f == TC_FUX ? s->p.nx - 1 : s->p.nx;
+      }
+      PROF(TC_S_XSOLVE, tck_launch_xsolve(&s->g, nf, xd, xo, xc, xr, xv, xl, xi0, xn, s->mixed, s->stream));
+    }
     for(int f = 0; f < nf; f++){
       tc_ls_t * L = &s->ls[f][rkstep];
       const int ux = f == TC_FUX;
       const int i0 = ux ? 2 : 1;
       const int n = ux ? s->p.nx - 1 : s->p.nx;
-      if(s->p.implicit[0]){
-        PROF(TC_S_XSOLVE, tck_launch_xsolve(&s->g, dlt[f], nxt[f], last == 0 ? cur[f] : NULL, L->xr, L->xv,
-            L->xl, i0, n, s->mixed, s->stream));
-      }
       if(s->p.implicit[1]){
         PROF(TC_S_YSOLVE, tck_launch_psolve(&s->g, 1, dlt[f], nxt[f], last == 1 ? cur[f] : NULL, L->yr, L->yv,
             L->yq1, L->ymeta, s->tp, 1, i0, n, s->mixed, s->stream));

@@ -327,13 +327,17 @@ static int num_cus(void){
 /* x lines of the implicit diffusion solve (solve_in_x, predict/linear_system.c:6-37).
  * d: increment in the field layout, rows i = i0 .. i0+n-1 of each (j, k) line.
  * final: store ucur + solution with halos (the field update of ux.c:544-556). */
+/* the x solves of up to TCK_XS_MAXF fields in one launch, one field per blockIdx.y: the
+ * fields' systems are independent, and on small grids a launch per field costs more than
+ * its solve */
+#define TCK_XS_MAXF 4
 struct XSolveArgs {
   tc_geom_t g;
-  void * d;              /* the increment: double, or float in the mixed step */
-  double * uout;         /* final pass: the new field (the increment's buffer itself in FP64) */
-  const double * ucur;
-  const double * r; const double * v; const double * l;
-  int i0, n;
+  void * d[TCK_XS_MAXF];              /* the increment: double, or float in the mixed step */
+  double * uout[TCK_XS_MAXF];         /* final pass: the new field (the increment's buffer itself in FP64) */
+  const double * ucur[TCK_XS_MAXF];
+  const double * r[TCK_XS_MAXF]; const double * v[TCK_XS_MAXF]; const double * l[TCK_XS_MAXF];
+  int i0[TCK_XS_MAXF], n[TCK_XS_MAXF];
 };
 
 /* LDS ordering between the lanes of one wavefront (the staging areas below are private to a
@@ -353,11 +357,11 @@ __device__ __forceinline__ void wave_sync(){
 template <int SEG, typename D>
 __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict__ dd,
     double * uout, const double * __restrict__ ucur, const double * __restrict__ rt,
-    const double * __restrict__ vt, const double * __restrict__ lt, double * sm, double (*tab)[TCK_WAVE * (SEG + 1)]){
+    const double * __restrict__ vt, const double * __restrict__ lt, const int i0, const int n,
+    double * sm, double (*tab)[TCK_WAVE * (SEG + 1)]){
   const tc_geom_t & g = A.g;
   const int w = threadIdx.x / TCK_WAVE;
   const int lane = threadIdx.x % TCK_WAVE;
-  const int n = A.n;
   for(int idx = threadIdx.x; idx < TCK_WAVE * SEG; idx += blockDim.x){
     const int row = idx < n ? idx : n - 1;
     tab[0][stage_pos<SEG>(idx)] = rt[row];
@@ -375,7 +379,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
   #define XS_FETCH(ln) { \
     const long long l_ = (ln); \
     if(l_ < nlines){ \
-      const long long b_ = tck_idx(g, A.i0, 1 + (int)(l_ % g.ny), 1 + (int)(l_ / g.ny)); \
+      const long long b_ = tck_idx(g, i0, 1 + (int)(l_ % g.ny), 1 + (int)(l_ / g.ny)); \
       _Pragma("unroll") for(int e = 0; e < SEG; e++){ \
         const int idx = lane + e * TCK_WAVE; \
         pd[e] = idx < n ? tck_ld(&dd[b_ + idx]) : D(0.); \
@@ -398,7 +402,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
       }
     }
     XS_FETCH(line + stride);
-    const long long base = tck_idx(g, A.i0, j, k);
+    const long long base = tck_idx(g, i0, j, k);
     wave_sync();
     /* the table offset is opaque to the compiler, so the reads stay inside the loop */
     int toff = lane * (SEG + 1);
@@ -415,7 +419,7 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
         #pragma unroll
         for(int e = 0; e < SEG; e++){
           const int idx = lane + e * TCK_WAVE;
-          if(idx < n) tck_store_halo(uout, g, A.i0 + idx, j, k, u[e] + sm[stage_pos<SEG>(idx)], false);
+          if(idx < n) tck_store_halo(uout, g, i0 + idx, j, k, u[e] + sm[stage_pos<SEG>(idx)], false);
         }
       }else{
         #pragma unroll
@@ -434,7 +438,9 @@ template <int SEG, typename D>
 __global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
   __shared__ double xs_sm[TCK_BLOCK / TCK_WAVE][TCK_WAVE * (SEG + 1)];
   __shared__ double xs_tab[3][TCK_WAVE * (SEG + 1)];
-  xsolve_body<SEG, D>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.l, xs_sm[threadIdx.x / TCK_WAVE], xs_tab);
+  const int f = blockIdx.y;
+  xsolve_body<SEG, D>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.l[f], A.i0[f], A.n[f],
+      xs_sm[threadIdx.x / TCK_WAVE], xs_tab);
 }
 
 /* periodic lines in y (solve_in_y, :39-70) or z (solve_in_z, :72-103).  A block holds TI
@@ -797,14 +803,15 @@ static int seg_for(int n){
 }
 
 template <int SEG>
-static int launch_xsolve_t(const XSolveArgs & A, int mixed, hipStream_t stream){
+static int launch_xsolve_t(const XSolveArgs & A, int nf, int mixed, hipStream_t stream){
   const long long nlines = (long long)A.g.ny * A.g.nz;
   const int wpb = TCK_BLOCK / TCK_WAVE;
   long long nblocks = (nlines + wpb - 1) / wpb;
   const long long cap = (long long)num_cus() * 8;
   if(nblocks > cap) nblocks = cap;
-  if(mixed) hipLaunchKernelGGL((tck_xsolve<SEG, float>), dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
-  else hipLaunchKernelGGL((tck_xsolve<SEG, double>), dim3((unsigned)nblocks), dim3(TCK_BLOCK), 0, stream, A);
+  const dim3 grd((unsigned)nblocks, (unsigned)nf);
+  if(mixed) hipLaunchKernelGGL((tck_xsolve<SEG, float>), grd, dim3(TCK_BLOCK), 0, stream, A);
+  else hipLaunchKernelGGL((tck_xsolve<SEG, double>), grd, dim3(TCK_BLOCK), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -853,17 +860,26 @@ int tck_launch_factor(const double * dt, const int dirs[3], int nx, int ny, int 
   return 0;
 }
 
-int tck_launch_xsolve(const tc_geom_t * g, void * d, double * uout, const double * ucur,
-    const double * r, const double * v, const double * l, int i0, int n, int mixed, hipStream_t stream){
-  if(n > TCK_WAVE * XS_MAXSEG || n < 2) return (int)hipErrorInvalidValue;
+int tck_launch_xsolve(const tc_geom_t * g, int nf, void * const * d, double * const * uout,
+    const double * const * ucur, const double * const * r, const double * const * v,
+    const double * const * l, const int * i0, const int * n, int mixed, hipStream_t stream){
+  if(nf < 1 || nf > TCK_XS_MAXF) return (int)hipErrorInvalidValue;
   XSolveArgs A;
-  A.g = *g; A.d = d; A.uout = uout; A.ucur = ucur; A.r = r; A.v = v; A.l = l; A.i0 = i0; A.n = n;
-  switch(seg_for(n)){
-    case 1: return launch_xsolve_t<1>(A, mixed, stream);
-    case 2: return launch_xsolve_t<2>(A, mixed, stream);
-    case 4: return launch_xsolve_t<4>(A, mixed, stream);
-    case 8: return launch_xsolve_t<8>(A, mixed, stream);
-    default: return launch_xsolve_t<16>(A, mixed, stream);
+  A.g = *g;
+  int nmax = 0;
+  for(int f = 0; f < nf; f++){
+    if(n[f] > TCK_WAVE * XS_MAXSEG || n[f] < 2) return (int)hipErrorInvalidValue;
+    A.d[f] = d[f]; A.uout[f] = uout[f]; A.ucur[f] = ucur[f];
+    A.r[f] = r[f]; A.v[f] = v[f]; A.l[f] = l[f]; A.i0[f] = i0[f]; A.n[f] = n[f];
+    if(n[f] > nmax) nmax = n[f];
+  }
+  /* the segment count of the longest line: shorter lines leave their last lanes idle */
+  switch(seg_for(nmax)){
+    case 1: return launch_xsolve_t<1>(A, nf, mixed, stream);
+    case 2: return launch_xsolve_t<2>(A, nf, mixed, stream);
+    case 4: return launch_xsolve_t<4>(A, nf, mixed, stream);
+    case 8: return launch_xsolve_t<8>(A, nf, mixed, stream);
+    default: return launch_xsolve_t<16>(A, nf, mixed, stream);
   }
 }
 
