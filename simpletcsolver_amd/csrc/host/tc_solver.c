@@ -813,34 +813,42 @@ static int predict(tc_solver_t * s, int rkstep){
   if(any_implicit){
     int last = 0;
     for(int d = 0; d < 3; d++) if(s->p.implicit[d]) last = d;
-    /* every field's solves run x, y, z in order; the fields are independent, so the x solves
-     * of all fields go in one launch */
+    /* every field's solves run x, y, z in order; the fields are independent, so each
+     * direction's solves of all fields go in one launch */
+    void * xd[4]; double * xo[4]; const double * xc[4];
+    const double * xr[4]; const double * xv[4]; const double * xl[4]; const double * xq[4];
+    const double * xm[4];
+    int xi0[4], xn[4];
+    for(int f = 0; f < nf; f++){
+      xd[f] = dlt[f]; xo[f] = nxt[f];
+      xi0[f] = f == TC_FUX ? 2 : 1;
+      xn[f] = f == TC_FUX ? s->p.nx - 1 : s->p.nx;
+    }
     if(s->p.implicit[0]){
-      void * xd[4]; double * xo[4]; const double * xc[4];
-      const double * xr[4]; const double * xv[4]; const double * xl[4];
-      int xi0[4], xn[4];
       for(int f = 0; f < nf; f++){
         tc_ls_t * L = &s->ls[f][rkstep];
-        xd[f] = dlt[f]; xo[f] = nxt[f]; xc[f] = last == 0 ? cur[f] : NULL;
+        xc[f] = last == 0 ? cur[f] : NULL;
         xr[f] = L->xr; xv[f] = L->xv; xl[f] = L->xl;
-        xi0[f] = f == TC_FUX ? 2 : 1;
-        xn[f] = f == TC_FUX ? s->p.nx - 1 : s->p.nx;
       }
       PROF(TC_S_XSOLVE, tck_launch_xsolve(&s->g, nf, xd, xo, xc, xr, xv, xl, xi0, xn, s->mixed, s->stream));
     }
-    for(int f = 0; f < nf; f++){
-      tc_ls_t * L = &s->ls[f][rkstep];
-      const int ux = f == TC_FUX;
-      const int i0 = ux ? 2 : 1;
-      const int n = ux ? s->p.nx - 1 : s->p.nx;
-      if(s->p.implicit[1]){
-        PROF(TC_S_YSOLVE, tck_launch_psolve(&s->g, 1, dlt[f], nxt[f], last == 1 ? cur[f] : NULL, L->yr, L->yv,
-            L->yq1, L->ymeta, s->tp, 1, i0, n, s->mixed, s->stream));
+    if(s->p.implicit[1]){
+      for(int f = 0; f < nf; f++){
+        tc_ls_t * L = &s->ls[f][rkstep];
+        xc[f] = last == 1 ? cur[f] : NULL;
+        xr[f] = L->yr; xv[f] = L->yv; xq[f] = L->yq1; xm[f] = L->ymeta;
       }
-      if(s->p.implicit[2]){
-        PROF(TC_S_ZSOLVE, tck_launch_psolve(&s->g, 2, dlt[f], nxt[f], cur[f], L->zr, L->zv, L->zq1, L->zmeta, 1, 0,
-            i0, n, s->mixed, s->stream));
+      PROF(TC_S_YSOLVE, tck_launch_psolve(&s->g, 1, nf, xd, xo, xc, xr, xv, xq, xm, s->tp, 1, xi0, xn,
+          s->mixed, s->stream));
+    }
+    if(s->p.implicit[2]){
+      for(int f = 0; f < nf; f++){
+        tc_ls_t * L = &s->ls[f][rkstep];
+        xc[f] = cur[f];
+        xr[f] = L->zr; xv[f] = L->zv; xq[f] = L->zq1; xm[f] = L->zmeta;
       }
+      PROF(TC_S_ZSOLVE, tck_launch_psolve(&s->g, 2, nf, xd, xo, xc, xr, xv, xq, xm, 1, 0, xi0, xn,
+          s->mixed, s->stream));
     }
   }
   for(int f = 0; f < nf; f++) s->cur[f] = 1 - s->cur[f];

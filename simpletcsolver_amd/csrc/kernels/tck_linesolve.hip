@@ -457,17 +457,18 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
 #define PS_MAXM 32
 #define PS_PRE 8
 
+/* the solves of up to TCK_XS_MAXF fields in one direction, one field per blockIdx.z */
 struct PSolveArgs {
   tc_geom_t g;
-  void * d;              /* the increment: double, or float in the mixed step */
-  double * uout;         /* last pass: the new field (the increment's buffer itself in FP64) */
-  const double * ucur;
-  const double * r; const double * v; const double * q1;
-  const double * meta;  /* TC_PMETA rows [7][tstride] indexed by i (y) or [7] (z) */
+  void * d[TCK_XS_MAXF];              /* the increment: double, or float in the mixed step */
+  double * uout[TCK_XS_MAXF];         /* last pass: the new field (the increment's buffer itself in FP64) */
+  const double * ucur[TCK_XS_MAXF];
+  const double * r[TCK_XS_MAXF]; const double * v[TCK_XS_MAXF]; const double * q1[TCK_XS_MAXF];
+  const double * meta[TCK_XS_MAXF];  /* TC_PMETA rows [7][tstride] indexed by i (y) or [7] (z) */
   long long tstride;    /* table stride between rows (tp for y, 1 for z) */
   int tcol;             /* 1: tables indexed by i as well (y) */
   int dir;              /* 1: y, 2: z */
-  int i0, isz;
+  int i0[TCK_XS_MAXF], isz[TCK_XS_MAXF];
 };
 
 /* factor rows row0+e0 .. row0+e0+PS_PRE-1 of table t: rows below jc from the table (loaded
@@ -498,7 +499,8 @@ __device__ __forceinline__ void ps_rows(Q * out, const double * __restrict__ t, 
 template <typename D, int MAXM>
 __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict__ d, double * uout,
     const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
-    const double * __restrict__ q1t, const double * __restrict__ meta, double * ps_sm){
+    const double * __restrict__ q1t, const double * __restrict__ meta, const int ib0, const int isz,
+    double * ps_sm){
   const tc_geom_t & g = A.g;
   const int TI = blockDim.x;
   const int S = blockDim.y;
@@ -507,7 +509,7 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   /* tiles start at i = 1 (a 128 B line boundary); rows below i0 are skipped (ux: i0 = 2) */
   const int i = 1 + blockIdx.x * TI + ti;
   const int other = 1 + blockIdx.y;    /* k for y lines, j for z lines */
-  const bool valid = i >= A.i0 && i < A.i0 + A.isz;
+  const bool valid = i >= ib0 && i < ib0 + isz;
   const int n = A.dir == 1 ? g.ny : g.nz;
   const int m = n - 1;
   const int seg = (m + S - 1) / S;
@@ -515,7 +517,7 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   int cnt = m - row0;
   cnt = cnt < 0 ? 0 : (cnt > seg ? seg : cnt);
   const long long step = A.dir == 1 ? (long long)g.px : g.plane;
-  const int ic = valid ? i : A.i0;
+  const int ic = valid ? i : ib0;
   const long long base = A.dir == 1 ? tck_idx(g, ic, 1, other) : tck_idx(g, ic, other, 1);
   const int col = A.tcol ? ic : 0;
   const long long ts = A.tstride;
@@ -668,7 +670,9 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
 template <typename D, int MAXM>
 __global__ void __launch_bounds__(512) tck_psolve(const PSolveArgs A){
   extern __shared__ double ps_sm[];
-  psolve_body<D, MAXM>(A, (D *)A.d, A.uout, A.ucur, A.r, A.v, A.q1, A.meta, ps_sm);
+  const int f = blockIdx.z;
+  psolve_body<D, MAXM>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.q1[f], A.meta[f],
+      A.i0[f], A.isz[f], ps_sm);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -883,12 +887,21 @@ int tck_launch_xsolve(const tc_geom_t * g, int nf, void * const * d, double * co
   }
 }
 
-int tck_launch_psolve(const tc_geom_t * g, int dir, void * d, double * uout, const double * ucur,
-    const double * r, const double * v, const double * q1, const double * meta,
-    long long tstride, int tcol, int i0, int isz, int mixed, hipStream_t stream){
+int tck_launch_psolve(const tc_geom_t * g, int dir, int nf, void * const * d, double * const * uout,
+    const double * const * ucur, const double * const * r, const double * const * v,
+    const double * const * q1, const double * const * meta, long long tstride, int tcol,
+    const int * i0, const int * isz, int mixed, hipStream_t stream){
+  if(nf < 1 || nf > TCK_XS_MAXF) return (int)hipErrorInvalidValue;
   PSolveArgs A;
-  A.g = *g; A.d = d; A.uout = uout; A.ucur = ucur; A.r = r; A.v = v; A.q1 = q1; A.meta = meta;
-  A.tstride = tstride; A.tcol = tcol; A.dir = dir; A.i0 = i0; A.isz = isz;
+  A.g = *g;
+  A.tstride = tstride; A.tcol = tcol; A.dir = dir;
+  int span = 0;   /* tiles cover i = 1 .. max(i0+isz-1) */
+  for(int f = 0; f < nf; f++){
+    A.d[f] = d[f]; A.uout[f] = uout[f]; A.ucur[f] = ucur[f];
+    A.r[f] = r[f]; A.v[f] = v[f]; A.q1[f] = q1[f]; A.meta[f] = meta[f];
+    A.i0[f] = i0[f]; A.isz[f] = isz[f];
+    if(i0[f] + isz[f] - 1 > span) span = i0[f] + isz[f] - 1;
+  }
   const int n = dir == 1 ? g->ny : g->nz;
   const int m = n - 1;
   /* the mixed step keeps twice the rows per thread (FP32 registers) so that a row access of
@@ -902,9 +915,9 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, void * d, double * uout, con
   if(TI * S > 512) return (int)hipErrorInvalidValue;
   const int nother = dir == 1 ? g->nz : g->ny;
   const size_t lds = sizeof(double) * (size_t)(2 * S * TI + 3 * TI);
-  const int span = i0 + isz - 1;   /* tiles cover i = 1 .. i0+isz-1 */
-  if(mixed) hipLaunchKernelGGL((tck_psolve<float, 2 * PS_MAXM>), dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
-  else hipLaunchKernelGGL((tck_psolve<double, PS_MAXM>), dim3((span + TI - 1) / TI, nother), dim3(TI, S), lds, stream, A);
+  const dim3 grd((span + TI - 1) / TI, nother, nf);
+  if(mixed) hipLaunchKernelGGL((tck_psolve<float, 2 * PS_MAXM>), grd, dim3(TI, S), lds, stream, A);
+  else hipLaunchKernelGGL((tck_psolve<double, PS_MAXM>), grd, dim3(TI, S), lds, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
