@@ -775,11 +775,13 @@ static int set_lds(const void * fn, size_t bytes){
 }
 
 /* Persistent launches, FP64 only (the FP32 forms, at 16 elements per thread, spill with the
- * prefetch registers): the kernels are compiled for FFT_EPT / 2 elements per thread per pass,
- * the count LDS-bound FP64 tiles take; a launch qualifies when its threads cover the tile at
- * that count and it has at least four tiles per resident block.  Returns 1 when launched (the caller checks the launch), 0 when the caller should
- * launch the one-tile form, < 0 when the LDS limit could not be raised.  Resident blocks per CU are queried once per (kernel, threads, LDS), so launches
- * recorded into a graph make no query. */
+ * prefetch registers).  The persistent kernels are compiled for FFT_EPT / 2 elements per
+ * thread per pass, the count the LDS-bound FP64 tiles take; a launch qualifies when its
+ * threads cover the tile at that count and it has at least four tiles per resident block.
+ * Returns 1 when launched (the caller checks the launch), 0 when the caller should launch the
+ * one-tile form, < 0 when the LDS limit could not be raised.  Resident blocks per CU are
+ * queried once per (kernel, threads, LDS), so launches recorded into a graph make no query. */
+#define PERSIST_EPT (FFT_EPT / 2)
 #define PERSIST_CACHE 16
 static const void * g_pc_fn[PERSIST_CACHE];
 static int g_pc_nt[PERSIST_CACHE], g_pc_per[PERSIST_CACHE];
@@ -789,9 +791,9 @@ static int g_pc_n = 0;
 }   /* extern "C" */
 
 template <typename Args>
-static int launch_persistent(void (*kern)(const Args), int eptk, int maxnt, const Args & A, int nt,
-    size_t lds, long long elems, long long ntiles, hipStream_t stream){
-  if((long long)nt * eptk < elems || nt > maxnt) return 0;
+static int launch_persistent(void (*kern)(const Args), const Args & A, int nt, size_t lds,
+    long long elems, long long ntiles, hipStream_t stream){
+  if((long long)nt * PERSIST_EPT < elems || nt > FFT_MAXTHREADS) return 0;
   const void * fn = (const void *)kern;
   int per = 0;
   for(int q = 0; q < g_pc_n; q++) if(g_pc_fn[q] == fn && g_pc_nt[q] == nt && g_pc_lds[q] == lds) per = g_pc_per[q];
@@ -830,7 +832,7 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
 #if TCK_DIVY_PF > 0
   {
     const long long el = (long long)A.ti * A.P.M;
-    const int r = mixed ? 0 : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, FFT_EPT / 2>, FFT_EPT / 2, FFT_MAXTHREADS, A, nt, lds, el, ntiles, stream);
+    const int r = mixed ? 0 : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT>, A, nt, lds, el, ntiles, stream);
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
@@ -861,7 +863,7 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
 #if TCK_ZFFT_PF > 0
   {
     const long long el = (long long)A.tz * nz;
-    const int r = mixed ? 0 : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, FFT_EPT / 2>, FFT_EPT / 2, FFT_MAXTHREADS, A, nt, lds, el, ntiles, stream);
+    const int r = mixed ? 0 : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT>, A, nt, lds, el, ntiles, stream);
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
@@ -893,7 +895,7 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
 #if TCK_YIFFT_PF > 0
   {
     const long long el = (long long)A.ti * A.P.M;
-    const int r = mixed ? 0 : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, FFT_EPT / 2>, FFT_EPT / 2, FFT_MAXTHREADS, A, nt, lds, el, ntiles, stream);
+    const int r = mixed ? 0 : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT>, A, nt, lds, el, ntiles, stream);
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
