@@ -77,14 +77,21 @@ struct PredictArgs {
 #define TCK_SECTION_FENCE() asm volatile("" ::: "memory")
 #define CT(t, ii) (smt[(t) * TW + (ii) - ib])
 
+/* the FP64 value of a cell's own field: from the staged plane when it is staged in FP64,
+ * else from the field in memory (the window of the mixed step stages FP32) */
+template <typename FT>
+__device__ __forceinline__ double own_value(const FT * staged, int of, const double * field, long long o){
+  return sizeof(FT) == 8 ? double(staged[of]) : field[o];
+}
+
 /* body with restrict-qualified pointers, so that the loads of each field section are free
  * to move ahead of the stores of the previous one */
-template <typename R, typename PT, int TW>
+template <typename R, typename PT, typename FT, int TW>
 __device__ __forceinline__ void predict_body(const PredictArgs & A,
-    const double * __restrict__ Fux_m, const double * __restrict__ Fux_0, const double * __restrict__ Fux_p,
-    const double * __restrict__ Fuy_m, const double * __restrict__ Fuy_0, const double * __restrict__ Fuy_p,
-    const double * __restrict__ Fuz_m, const double * __restrict__ Fuz_0, const double * __restrict__ Fuz_p,
-    const double * __restrict__ Ft_m, const double * __restrict__ Ft_0, const double * __restrict__ Ft_p,
+    const FT * __restrict__ Fux_m, const FT * __restrict__ Fux_0, const FT * __restrict__ Fux_p,
+    const FT * __restrict__ Fuy_m, const FT * __restrict__ Fuy_0, const FT * __restrict__ Fuy_p,
+    const FT * __restrict__ Fuz_m, const FT * __restrict__ Fuz_0, const FT * __restrict__ Fuz_p,
+    const FT * __restrict__ Ft_m, const FT * __restrict__ Ft_0, const FT * __restrict__ Ft_p,
     const PT * __restrict__ Fp_m, const PT * __restrict__ Fp_0, const int of, const int fsy,
     R * __restrict__ dxo, R * __restrict__ dyo, R * __restrict__ dzo, R * __restrict__ dto,
     double * __restrict__ nxo, double * __restrict__ nyo, double * __restrict__ nzo, double * __restrict__ nto,
@@ -182,7 +189,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbx) b = sbv ? sbv[0] : sbx[o];
     const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sax) tck_st(&sax[o], a);
-    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, Fux_0[of] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, own_value(Fux_0, of, A.ux, kb0 + o) + (double)delta, false);
     else tck_st(&dxo[o], delta);
   }
 
@@ -270,7 +277,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sby) b = sbv ? sbv[1] : sby[o];
     const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(say) tck_st(&say[o], a);
-    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, Fuy_0[of] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, own_value(Fuy_0, of, A.uy, kb0 + o) + (double)delta, false);
     else tck_st(&dyo[o], delta);
   }
 
@@ -338,7 +345,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbz) b = sbv ? sbv[2] : sbz[o];
     const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(saz) tck_st(&saz[o], a);
-    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, Fuz_0[of] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, own_value(Fuz_0, of, A.uz, kb0 + o) + (double)delta, false);
     else tck_st(&dzo[o], delta);
   }
 
@@ -387,7 +394,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbt) b = sbv ? sbv[3] : sbt[o];
     const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sat) tck_st(&sat[o], a);
-    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, Ft_0[of] + (double)delta, false);
+    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, own_value(Ft_0, of, A.t, kb0 + o) + (double)delta, false);
     else tck_st(&dto[o], delta);
   }
 }
@@ -423,7 +430,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
     #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
     #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
-    predict_body<R, double, TCK_TW>(A, A.ux + kb0, A.ux + kb0 + pl, A.ux + kb0 + 2 * pl,
+    predict_body<R, double, double, TCK_TW>(A, A.ux + kb0, A.ux + kb0 + pl, A.ux + kb0 + 2 * pl,
         A.uy + kb0, A.uy + kb0 + pl, A.uy + kb0 + 2 * pl,
         A.uz + kb0, A.uz + kb0 + pl, A.uz + kb0 + 2 * pl,
         tf + kb0, tf + kb0 + pl, tf + kb0 + 2 * pl,
@@ -452,18 +459,23 @@ __device__ __forceinline__ void tck_lds_barrier(){
  * (34 x 10 cells) in a ring of plane buffers, so that every plane is read from memory once
  * per block (1.33 times the tile) instead of once per stencil plane, and the stencils read
  * LDS.  The next plane's loads are issued into registers before the current plane's
- * stencils run.  49 KiB of LDS with FP64 coefficient rows, 41 KiB in the mixed step: three
- * blocks per CU.  Outputs and RK sources are addressed as in tck_predict. */
+ * stencils run.  The planes are staged in the arithmetic type: 49 KiB of LDS in FP64, 25 KiB
+ * in the mixed step; three blocks per CU either way (FP64: its LDS; FP32: its registers,
+ * held to three waves per SIMD, at which it runs faster than at four with spills).  Outputs
+ * and RK sources are addressed as in tck_predict. */
 #define TCK_WX 32                          /* tile width in i */
 #define TCK_WY 8                           /* tile height in j */
 #define TCK_WTW (TCK_WX + 2)               /* window columns (and coefficient-row pitch) */
 #define TCK_WPLANE ((TCK_WY + 2) * TCK_WTW)
 #define TCK_WPF ((TCK_WPLANE + TCK_BLOCK - 1) / TCK_BLOCK)   /* window elements per thread */
 
+
 template <typename R>
 __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) tck_predict_w(const PredictArgs A){
   __shared__ R smt[T_NPRED * TCK_WTW];
-  __shared__ double wf[4][3][TCK_WPLANE];   /* ux uy uz T: a ring of three planes */
+  __shared__ R wf[4][3][TCK_WPLANE];        /* ux uy uz T: a ring of three planes, in the
+                                             * arithmetic type (the stencils read them only
+                                             * through R()) */
   __shared__ R wq[2][TCK_WPLANE];           /* p: two planes, in the arithmetic type (the
                                              * stencils read p only through R()) */
   int bx, by, bz;
@@ -496,6 +508,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     const int r = e / TCK_WTW, cc = e - r * TCK_WTW;
     woff[q] = (e < TCK_WPLANE && r < nrow && cc < ncol) ? (long long)(ib + cc + TC_XOFF) + px * (jw + r) : -1;
   }
+  /* FP64 until the plane is put in LDS: a conversion here would wait for the loads */
   double pre[5][TCK_WPF];
   /* the cell's RK sources of the previous substep (substeps 1 and 2): loaded with the next
    * plane in the mixed step; the FP64 form has no registers to spare for them and the body
@@ -516,7 +529,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     _Pragma("unroll") for(int q = 0; q < TCK_WPF; q++){ \
       const int e = tid + q * TCK_BLOCK; \
       if(e < TCK_WPLANE){ \
-        _Pragma("unroll") for(int f = 0; f < 4; f++) wf[f][slot][e] = pre[f][q]; \
+        _Pragma("unroll") for(int f = 0; f < 4; f++) wf[f][slot][e] = R(pre[f][q]); \
         wq[pslot][e] = R(pre[4][q]); \
       } \
     }
@@ -540,7 +553,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
     #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
     #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
-    predict_body<R, R, TCK_WTW>(A, wf[0][sm], wf[0][s0], wf[0][sp], wf[1][sm], wf[1][s0], wf[1][sp],
+    predict_body<R, R, R, TCK_WTW>(A, wf[0][sm], wf[0][s0], wf[0][sp], wf[1][sm], wf[1][s0], wf[1][sp],
         wf[2][sm], wf[2][s0], wf[2][sp], wf[ft][sm], wf[ft][s0], wf[ft][sp],
         wq[pm], wq[1 - pm], of, TCK_WTW,
         RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
