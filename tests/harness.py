@@ -40,8 +40,10 @@ def rel_l2(a, b):
 
 
 def jmean(a, case):
-    """J-weighted mean of a centre field in the NPY layout (interior x cells)."""
-    o = Oracle(case["nx"], case["ny"], case["nz"], case["xf"], case["xc"], **case["kw"])
+    """J-weighted mean of a centre field in the NPY layout (interior x cells).  J = hx hy hz
+    depends on y and z only through the constant factors ly / ny and lz / nz, which cancel in
+    a normalised mean, so the weights come from an oracle of 4 x 4 cells in y and z."""
+    o = Oracle(case["nx"], 4, 4, case["xf"], case["xc"], **case["kw"])
     jd = o.array("jdxc")[1:-1].copy()
     o.close()
     w = np.broadcast_to(jd, a[:, :, 1:-1].shape)

@@ -1,0 +1,68 @@
+"""GPU against the CPU oracle at the benchmark configurations themselves:
+
+* 64 x 64 x 64 Taylor-Couette, Re = 1000, explicit diffusion, no scalar, laminar profile plus
+  seeded noise of amplitude 1e-2, fixed dt = 1e-3: after 10 steps and after 100;
+* 256 x 512 x 512 Taylor-Couette, Re = 5000, implicit radial diffusion, passive scalar with
+  T = 1 and 0 at the walls starting from the conduction profile, noise 1e-2, CFL dt with
+  coefficients 1: after 2 steps (the oracle's share of the run, about 20 s on the host).
+
+Radius ratio 0.714 and a tanh-clustered radial grid in both: stretch 2 for the large case,
+stretch 1 for the 64-cell one, whose explicit diffusion at dt = 1e-3 needs its smallest
+radial cell above about 2.4e-3 (with stretch 2 the wall cells are 9.7e-4 wide, the diffusive
+limit 1.6e-4, and both codes blow up alike; with stretch 1 they are 3.5e-3 wide); the stretch
+is not part of either configuration, nor are the theta and z lengths, which take the harness
+defaults.  Bar: each field within 1e-10 relative L2 (p and psi modulo their J-weighted mean,
+psi within 1e-8 as in the ten-step tests; after 100 steps psi's difference within 1e-10 of
+the norm of p), equal dt, and max |div u| <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+from harness import gpu_for, make_case, modulo_mean, oracle_for
+from test_gpu_parity import VEL, _cmp
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(o, s, case, names):
+    _cmp(o, s, names, case, 1e-10, gauge=("p", "psi"))
+    assert s.divergence() <= 1e-12
+
+
+def test_config_64cubed_explicit_100_steps(need_gpu):
+    case = make_case(nx=64, ny=64, nz=64, Re=1000.0, implicit=(0, 0, 0), scalar=0, amp=1e-2,
+                     dt_fixed=1e-3, stretch=1.0)
+    o, s = oracle_for(case), gpu_for(case)
+    for n in range(100):
+        dto = o.integrate()
+        dts = s.step()
+        assert dts == dto == 1e-3, (n, dts, dto)
+        if n + 1 == 10:
+            print("after 10 steps:")
+            _compare(o, s, case, VEL + ("p", "psi"))
+    # after 100 steps the noise has decayed and psi, the last substep's pressure increment, is
+    # small next to p: its difference is measured against p, the field it updates
+    print("after 100 steps:")
+    _compare(o, s, case, VEL + ("p",))
+    op = o.npy_fields()
+    dpsi = modulo_mean(s.get_field("psi"), case) - modulo_mean(op["psi"], case)
+    ratio = np.linalg.norm(dpsi) / np.linalg.norm(modulo_mean(op["p"], case))
+    print(f"psi difference / |p|: {ratio:.3e}")
+    assert ratio <= 1e-10
+    o.close()
+    s.close()
+
+
+def test_config_256x512x512_implicit_scalar_2_steps(need_gpu):
+    case = make_case(nx=256, ny=512, nz=512, Re=5000.0, Pr=1.0, implicit=(1, 0, 0), scalar=1,
+                     amp=1e-2, t_walls=(1.0, 0.0))
+    o = oracle_for(case)
+    s = gpu_for(case)
+    del case["fields"]
+    for n in range(2):
+        dto = o.integrate()
+        dts = s.step()
+        assert abs(dts - dto) <= 1e-15 * abs(dto), (n, dts, dto)
+    _compare(o, s, case, VEL + ("t", "p", "psi"))
+    o.close()
+    s.close()
