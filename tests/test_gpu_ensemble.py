@@ -21,7 +21,7 @@ def _specs():
     for m, (nx, ny, nz, eta, Re, curved, scalar, implicit) in enumerate([
             (16, 12, 8, 0.714, 300.0, True, 0, (0, 0, 0)),
             (21, 24, 12, 0.55, 800.0, True, 1, (0, 0, 0)),
-            (12, 16, 16, 0.9, 500.0, False, 0, (0, 0, 0)),
+            (12, 16, 16, 0.9, 500.0, False, 0, (0, 0, 1)),
             (18, 12, 24, 0.6, 200.0, True, 1, (1, 1, 0))]):
         lx = (1.0 / eta - 1.0) if curved else 1.0
         xf, xc = icgen.grid(nx, lx, curved, 1.5)
