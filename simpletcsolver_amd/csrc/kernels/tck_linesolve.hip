@@ -31,8 +31,9 @@
  *   last pivot c_{n-1} - l_{n-1} v_{n-2}: r_{n-1} = 1 / pivot, or 0 when |pivot| <= eps.
  * Tables: r[0] holds c0 itself (row 0 divides), l[i] = l_i, v[i] = v_i.
  * ------------------------------------------------------------------------------------- */
-__device__ void factor_sys(int n, const double * lc, const double * cc, const double * uc,
-    double pf, long long stride, double * r, double * v, double * l){
+__device__ void factor_sys(int n, const double * __restrict__ lc, const double * __restrict__ cc,
+    const double * __restrict__ uc, double pf, long long stride, double * __restrict__ r,
+    double * __restrict__ v, double * __restrict__ l){
   /* tdm coefficients as solve_in_x builds them (predict/linear_system.c:28-32) */
   #define TL(i) (- pf * lc[i])
   #define TC(i) (1. - pf * cc[i])
@@ -41,6 +42,8 @@ __device__ void factor_sys(int n, const double * lc, const double * cc, const do
   r[0] = TC(0);
   v[0] = vprev;
   l[0] = TL(0);
+  /* unrolled so that the coefficient loads of several rows are in flight together */
+  #pragma unroll 8
   for(int i = 1; i < n - 1; i++){
     const double val = 1. / (TC(i) - TL(i) * vprev);
     vprev = val * (TU(i));
@@ -66,7 +69,8 @@ __device__ void factor_sys(int n, const double * lc, const double * cc, const do
 #define TC_PMETA 7
 
 __device__ void factor_periodic(int n, double tl, double tc, double tu, long long stride,
-    double * r, double * v, double * q1, double * meta, long long mstride){
+    double * __restrict__ r, double * __restrict__ v, double * __restrict__ q1,
+    double * __restrict__ meta, long long mstride){
   const int m = n - 1;
   /* factors of the (n-1) leading block */
   double vprev = tu / tc;
@@ -91,7 +95,10 @@ __device__ void factor_periodic(int n, double tl, double tc, double tu, long lon
     v[stride * (m - 1)] = 0.;
   }
   /* q1 = A_{n-1}^{-1} (-l0, 0, ..., 0, -u_{n-2}) by the same gtsv arithmetic */
+  /* the loads of r and v below read rows written by the loop above; unrolled so that several
+   * rows' loads are in flight together, and the backward pass carries q1_{i+1} in a register */
   double qprev = 0.;
+  #pragma unroll 8
   for(int i = 0; i < m; i++){
     const double rhs = i == 0 ? - tl : i == m - 1 ? - tu : 0.;
     double q;
@@ -100,8 +107,12 @@ __device__ void factor_periodic(int n, double tl, double tc, double tu, long lon
     q1[stride * i] = q;
     qprev = q;
   }
+  double qnext = q1[stride * (m - 1)];
+  #pragma unroll 8
   for(int i = m - 2; i >= 0; i--){
-    q1[stride * i] -= v[stride * i] * q1[stride * (i + 1)];
+    const double q = q1[stride * i] - v[stride * i] * qnext;
+    q1[stride * i] = q;
+    qnext = q;
   }
   meta[0] = tc + tu * q1[0] + tl * q1[stride * (m - 1)];
   meta[mstride] = tl;
