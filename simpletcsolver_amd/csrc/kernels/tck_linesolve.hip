@@ -42,9 +42,22 @@ __device__ void factor_sys(int n, const double * __restrict__ lc, const double *
   r[0] = TC(0);
   v[0] = vprev;
   l[0] = TL(0);
-  /* unrolled so that the coefficient loads of several rows are in flight together */
-  #pragma unroll 8
-  for(int i = 1; i < n - 1; i++){
+  /* eight rows' coefficients are loaded together before their recurrence steps */
+  int i = 1;
+  for(; i + 8 <= n - 1; i += 8){
+    double tla[8], tca[8], tua[8];
+    #pragma unroll
+    for(int e = 0; e < 8; e++){ tla[e] = TL(i + e); tca[e] = TC(i + e); tua[e] = TU(i + e); }
+    #pragma unroll
+    for(int e = 0; e < 8; e++){
+      const double val = 1. / (tca[e] - tla[e] * vprev);
+      vprev = val * (tua[e]);
+      r[stride * (i + e)] = val;
+      v[stride * (i + e)] = vprev;
+      l[stride * (i + e)] = tla[e];
+    }
+  }
+  for(; i < n - 1; i++){
     const double val = 1. / (TC(i) - TL(i) * vprev);
     vprev = val * (TU(i));
     r[stride * i] = val;
@@ -95,21 +108,45 @@ __device__ void factor_periodic(int n, double tl, double tc, double tu, long lon
     v[stride * (m - 1)] = 0.;
   }
   /* q1 = A_{n-1}^{-1} (-l0, 0, ..., 0, -u_{n-2}) by the same gtsv arithmetic */
-  /* the loads of r and v below read rows written by the loop above; unrolled so that several
-   * rows' loads are in flight together, and the backward pass carries q1_{i+1} in a register */
-  double qprev = 0.;
-  #pragma unroll 8
-  for(int i = 0; i < m; i++){
-    const double rhs = i == 0 ? - tl : i == m - 1 ? - tu : 0.;
-    double q;
-    if(i == 0) q = rhs / tc;
-    else q = r[stride * i] * (rhs - tl * qprev);
+  /* the loads of r and v below read rows written by the loop above: eight rows' loads are
+   * issued together; the backward pass carries q1_{i+1} in a register */
+  double qprev = - tl / tc;   /* row 0 */
+  q1[0] = qprev;
+  int i = 1;
+  for(; i + 8 <= m; i += 8){
+    double ra[8];
+    #pragma unroll
+    for(int e = 0; e < 8; e++) ra[e] = r[stride * (i + e)];
+    #pragma unroll
+    for(int e = 0; e < 8; e++){
+      const double rhs = i + e == m - 1 ? - tu : 0.;
+      const double q = ra[e] * (rhs - tl * qprev);
+      q1[stride * (i + e)] = q;
+      qprev = q;
+    }
+  }
+  for(; i < m; i++){
+    const double rhs = i == m - 1 ? - tu : 0.;
+    const double q = r[stride * i] * (rhs - tl * qprev);
     q1[stride * i] = q;
     qprev = q;
   }
   double qnext = q1[stride * (m - 1)];
-  #pragma unroll 8
-  for(int i = m - 2; i >= 0; i--){
+  i = m - 2;
+  /* eight rows' loads before any of their stores, so that the rows do not wait one by one
+   * for a store to reach memory before the next load */
+  for(; i >= 7; i -= 8){
+    double qa[8], va[8];
+    #pragma unroll
+    for(int e = 0; e < 8; e++){ qa[e] = q1[stride * (i - e)]; va[e] = v[stride * (i - e)]; }
+    #pragma unroll
+    for(int e = 0; e < 8; e++){
+      const double q = qa[e] - va[e] * qnext;
+      q1[stride * (i - e)] = q;
+      qnext = q;
+    }
+  }
+  for(; i >= 0; i--){
     const double q = q1[stride * i] - v[stride * i] * qnext;
     q1[stride * i] = q;
     qnext = q;
