@@ -125,11 +125,13 @@ int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty);
 int tc_autotune(tc_solver_t * s, int reps, int * kb, int * ty, double * best_ms);
 
 /* no reference counterpart: n independent flows (any sizes, parameters and precisions, one
- * device) advanced together.  The members share one stream; the first tc_ensemble_step takes
- * one eager step and records each RK substep of the whole batch into a hipGraph (for both
+ * device) advanced together, spread over a few streams; the first tc_ensemble_step takes one
+ * eager step and records each RK substep of the whole batch into a hipGraph (for both
  * parities of the members' double buffers), every later step is three graph launches.
- * Members are ordinary solvers for set/get/diagnostics (tc_ensemble_member); after a member's
- * fields are replaced, or after it is stepped on its own, call tc_ensemble_invalidate. */
+ * Members are ordinary solvers for set/get/diagnostics (tc_ensemble_member).  A step records
+ * again by itself when a member changed in a way the graphs cannot see (fields replaced, a
+ * closure coupled, uncoupled or destroyed, launch shapes or dt parameters set, a member
+ * stepped on its own); tc_ensemble_invalidate drops the graphs explicitly. */
 typedef struct tc_ensemble_s tc_ensemble_t;
 int tc_ensemble_create(int n, const tc_params_t * params, const double * const * xf,
     const double * const * xc, tc_ensemble_t ** out);

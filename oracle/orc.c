@@ -94,6 +94,10 @@ void orc_rkcoefs(double out[9]){
   }
 }
 
+/* planted mistake (orc_set_mutation); ORC_MUT_NONE in every use but the pin tests */
+static int mut = ORC_MUT_NONE;
+void orc_set_mutation(int m){ mut = m; }
+
 static double nu_of(const orc_t * o){ return 1. / o->c.Re; }               /* compute_diffusivity.c:3-7 */
 static double kappa_of(const orc_t * o){ return 1. / o->c.Re / o->c.Pr; }  /* compute_diffusivity.c:9-13 */
 
@@ -402,6 +406,7 @@ int orc_update_boundaries(orc_t * o){
   bc_uz(o);
   bc_neumann(o, o->p);
   bc_t(o);
+  bc_neumann(o, o->psi);   /* boundary/psi.c:13-45 */
   return 0;
 }
 
@@ -1140,11 +1145,13 @@ static void update_field(orc_t * o, int rkstep, double dt, double diffusivity,
   const double * sa = src[0];
   const double * sb = src[1];
   const double * sg = src[2];
+  const double * wa = mut == ORC_MUT_AB_SWAP ? sb : sa;
+  const double * wb = mut == ORC_MUT_AB_SWAP ? sa : sb;
   #pragma omp parallel for schedule(static)
   for(size_t n = 0; n < nitems; n++){
-    d[n] = + coef_a * dt * sa[n] + coef_b * dt * sb[n] + coef_g * dt * sg[n];
+    d[n] = + coef_a * dt * wa[n] + coef_b * dt * wb[n] + coef_g * dt * sg[n];
   }
-  const double prefactor = 0.5 * rk_g(rkstep) * dt * diffusivity;
+  const double prefactor = 0.5 * (mut == ORC_MUT_PF_ALPHA ? coef_a : rk_g(rkstep)) * dt * diffusivity;
   if(o->c.implicit[0]) solve_x(o, fx * prefactor, lapx, d, isz);
   if(o->c.implicit[1]) solve_y(o, fy * prefactor, lapy, d, isz);
   if(o->c.implicit[2]) solve_z(o, fz * prefactor, &o->lapz, d, isz);
@@ -1152,9 +1159,11 @@ static void update_field(orc_t * o, int rkstep, double dt, double diffusivity,
 
 /* reset_srcs predict/main.c:8-25 (pointer swap a <-> b, zero a and g) */
 static void reset_srcs(double * s[3], size_t nitems){
-  double * tmp = s[0];
-  s[0] = s[1];
-  s[1] = tmp;
+  if(mut != ORC_MUT_NO_ROTATE){
+    double * tmp = s[0];
+    s[0] = s[1];
+    s[1] = tmp;
+  }
   memset(s[0], 0, sizeof(double) * nitems);
   memset(s[2], 0, sizeof(double) * nitems);
 }
@@ -1174,20 +1183,22 @@ int orc_predict(orc_t * o, int rkstep, double dt){
   if(o->c.scalar) rhs_t(o);
   const double nu = nu_of(o);
   /* update_ux: 2 f in x (ux.c:499-506) */
-  update_field(o, rkstep, dt, nu, o->srcux, o->lapx_ux, o->lapy_ux, NX - 1, 2., 1., 1.);
+  update_field(o, rkstep, dt, nu, o->srcux, o->lapx_ux, o->lapy_ux, NX - 1,
+      mut == ORC_MUT_UX_X_F ? 1. : 2., mut == ORC_MUT_UX_Y_2F ? 2. : 1., 1.);
   XLOOP { UX(i, j, k) += o->delta[CNTX(i, j, k)]; }
   bc_ux(o);
   /* update_uy: 2 f in y (uy.c:539-544) */
-  update_field(o, rkstep, dt, nu, o->srcuy, o->lapx_c, o->lapy_c, NX, 1., 2., 1.);
+  update_field(o, rkstep, dt, nu, o->srcuy, o->lapx_c, o->lapy_c, NX, 1., mut == ORC_MUT_UY_Y_F ? 1. : 2., 1.);
   CLOOP { UY(i, j, k) += o->delta[CNTC(i, j, k)]; }
   bc_uy(o);
   /* update_uz: 2 f in z (uz.c:449-454) */
-  update_field(o, rkstep, dt, nu, o->srcuz, o->lapx_c, o->lapy_c, NX, 1., 1., 2.);
+  update_field(o, rkstep, dt, nu, o->srcuz, o->lapx_c, o->lapy_c, NX,
+      mut == ORC_MUT_UZ_X_2F ? 2. : 1., 1., mut == ORC_MUT_UZ_Z_F ? 1. : 2.);
   CLOOP { UZ(i, j, k) += o->delta[CNTC(i, j, k)]; }
   bc_uz(o);
   if(o->c.scalar){
     /* update_t: 2 f in y is a reference quirk (t.c:346-349), kept */
-    update_field(o, rkstep, dt, kappa_of(o), o->srct, o->lapx_c, o->lapy_c, NX, 1., 2., 1.);
+    update_field(o, rkstep, dt, kappa_of(o), o->srct, o->lapx_c, o->lapy_c, NX, 1., mut == ORC_MUT_T_Y_F ? 1. : 2., 1.);
     CLOOP { T(i, j, k) += o->delta[CNTC(i, j, k)]; }
     bc_t(o);
   }
@@ -1373,14 +1384,15 @@ int orc_pressure(orc_t * o, int rkstep, double dt){
   const double hz = o->hz;
   CLOOP { P(i, j, k) += PSI(i, j, k); }
   const double prefactor = 0.5 * rk_g(rkstep) * dt * nu_of(o);
+  const double two = mut == ORC_MUT_P_F ? 1. : 2.;
   if(o->c.implicit[0]){
     CLOOP {
-      const double hx_xm = HXXF(i), hx_xp = HXXF(i + 1);
+      const double hx_xm = HXXF(mut == ORC_MUT_P_X_INDEX ? i + 1 : i), hx_xp = HXXF(i + 1);
       const double jd_xm = JDXF(i), jd_x0 = JDXC(i), jd_xp = JDXF(i + 1);
       const double l = jd_xm / hx_xm / hx_xm;
       const double u = jd_xp / hx_xp / hx_xp;
       const double c = - l - u;
-      P(i, j, k) -= 2. * prefactor / jd_x0 * (+ l * PSI(i - 1, j, k) + c * PSI(i, j, k) + u * PSI(i + 1, j, k));
+      P(i, j, k) -= two * prefactor / jd_x0 * (+ l * PSI(i - 1, j, k) + c * PSI(i, j, k) + u * PSI(i + 1, j, k));
     }
   }
   if(o->c.implicit[1]){
@@ -1390,16 +1402,16 @@ int orc_pressure(orc_t * o, int rkstep, double dt){
       const double l = jd / hy / hy;
       const double u = jd / hy / hy;
       const double c = - l - u;
-      P(i, j, k) -= 2. * prefactor / jd * (+ l * PSI(i, j - 1, k) + c * PSI(i, j, k) + u * PSI(i, j + 1, k));
+      P(i, j, k) -= two * prefactor / jd * (+ l * PSI(i, j - 1, k) + c * PSI(i, j, k) + u * PSI(i, j + 1, k));
     }
   }
-  if(o->c.implicit[2]){
+  if(o->c.implicit[2] && mut != ORC_MUT_P_NO_Z){
     CLOOP {
       const double jd = JDXC(i);
       const double l = jd / hz / hz;
       const double u = jd / hz / hz;
       const double c = - l - u;
-      P(i, j, k) -= 2. * prefactor / jd * (+ l * PSI(i, j, k - 1) + c * PSI(i, j, k) + u * PSI(i, j, k + 1));
+      P(i, j, k) -= two * prefactor / jd * (+ l * PSI(i, j, k - 1) + c * PSI(i, j, k) + u * PSI(i, j, k + 1));
     }
   }
   bc_neumann(o, o->p);

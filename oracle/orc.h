@@ -54,7 +54,7 @@ void orc_destroy(orc_t * o);
 double * orc_array(orc_t * o, const char * name, size_t * nitems);
 double orc_hz(const orc_t * o);
 
-int orc_update_boundaries(orc_t * o);   /* BCs + halos of ux uy uz p t (fluid/init.c:112-116) */
+int orc_update_boundaries(orc_t * o);   /* BCs + halos of ux uy uz p t (fluid/init.c:112-116) and psi */
 int orc_decide_dt(orc_t * o, double * dt);
 int orc_predict(orc_t * o, int rkstep, double dt);
 int orc_potential(orc_t * o, int rkstep, double dt);
@@ -79,6 +79,27 @@ int orc_gtsv(int n, const double * l, const double * c, const double * u, double
 int orc_gtsv_complex(int n, const double * l, const double * c, const double * u, double * q);
 int orc_tdm_periodic(int n, const double * l, const double * c, const double * u, double * q);
 void orc_rkcoefs(double out[9]);
+
+/* Planted mistakes, for the tests that show each pin of the oracle fails on the slip it is
+ * meant to catch.  ORC_MUT_NONE (the default) is the reference's algorithm; nothing but those
+ * tests sets another value.  The setting is process-wide. */
+enum {
+  ORC_MUT_NONE = 0,
+  ORC_MUT_UX_X_F,      /* ux implicit x solve with f instead of 2f (predict/ux.c:499-506) */
+  ORC_MUT_UY_Y_F,      /* uy implicit y solve with f instead of 2f (predict/uy.c:539-544) */
+  ORC_MUT_UZ_Z_F,      /* uz implicit z solve with f instead of 2f (predict/uz.c:449-454) */
+  ORC_MUT_T_Y_F,       /* T implicit y solve with f: the predict/t.c:346-349 quirk undone */
+  ORC_MUT_UX_Y_2F,     /* ux implicit y solve with 2f instead of f (predict/ux.c:508-524) */
+  ORC_MUT_UZ_X_2F,     /* uz implicit x solve with 2f instead of f (predict/uz.c:418-420) */
+  ORC_MUT_PF_ALPHA,    /* implicit prefactor from alpha instead of gamma (ux.c:496-497) */
+  ORC_MUT_AB_SWAP,     /* RK combination weighs a with beta and b with alpha (ux.c:476-494) */
+  ORC_MUT_NO_ROTATE,   /* reset_srcs zeroes a without the a/b pointer swap (predict/main.c:16-18) */
+  ORC_MUT_P_F,         /* update_pressure with f instead of 2f (update_pressure.c:66, 98, 129) */
+  ORC_MUT_P_NO_Z,      /* update_pressure leaves out the implicit z term (:165-167) */
+  ORC_MUT_P_X_INDEX,   /* update_pressure x term with HXXF(i+1) in the i-1 coefficient (:54-59) */
+  ORC_MUT_COUNT
+};
+void orc_set_mutation(int m);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,7 @@ def library():
     for name in ("orc_gtsv", "orc_gtsv_complex", "orc_tdm_periodic"):
         getattr(lib, name).argtypes = [ctypes.c_int, dp, dp, dp, dp]
     lib.orc_rkcoefs.argtypes = [dp]
+    lib.orc_set_mutation.argtypes = [ctypes.c_int]
     lib.orc_stat_collect.argtypes = [vp]
     lib.orc_stat_collect.restype = ctypes.c_size_t
     _LIB = lib
@@ -151,7 +152,7 @@ class Oracle:
 
     def load_fields(self, fields):
         """fields: dict of NPY-layout arrays (nz, ny, nx+1|nx+2) -> interior planes, then BCs."""
-        for name in ("ux", "uy", "uz", "p", "t"):
+        for name in ("ux", "uy", "uz", "p", "psi", "t"):
             if name in fields:
                 self.array(name)[1:-1, 1:-1, :] = fields[name]
         self._lib.orc_update_boundaries(self._h)
@@ -213,6 +214,17 @@ class Oracle:
 
     def stats(self):
         return {n: self.array(n).copy() for n in ("ux1", "ux2", "uy1", "uy2", "uz1", "uz2", "adv", "dif")}
+
+
+# planted mistakes of orc.h (ORC_MUT_*), in enum order; only the pin tests set one
+MUTATIONS = ("none", "ux_x_f", "uy_y_f", "uz_z_f", "t_y_f", "ux_y_2f", "uz_x_2f", "pf_alpha",
+             "ab_swap", "no_rotate", "p_f", "p_no_z", "p_x_index")
+
+
+def set_mutation(name):
+    """plant one of the mistakes of orc.h in every oracle of this process ("none": the
+    reference's algorithm)"""
+    library().orc_set_mutation(MUTATIONS.index(name))
 
 
 def fft_c2c(x, sign):

@@ -95,8 +95,16 @@ static unsigned short bf16(float x){
 
 int tc_cnn_destroy(tc_cnn_t * c){
   if(!c) return 0;
-  if(c->owner) c->owner->closure = NULL;
   hipSetDevice(c->device);
+  if(c->owner){
+    /* the coupled solver's kernels read the network's buffers on the solver's stream (and
+     * an ensemble's replays are ordered before that stream's later work): let them finish,
+     * then uncouple; the generation bump makes recorded graphs re-record without it */
+    if(c->owner->stream) hipStreamSynchronize(c->owner->stream);
+    c->owner->closure = NULL;
+    c->owner->gen++;
+    c->owner = NULL;
+  }
   if(c->stream) hipStreamSynchronize(c->stream);
   for(int l = 0; l < 3; l++){
     void * p[] = {c->w[l], c->b[l], c->wbf[l], c->mw[l], c->vw[l], c->mb[l], c->vb[l], c->gw[l], c->gb[l], c->wt[l]};
@@ -296,6 +304,7 @@ int tc_closure_attach(tc_solver_t * s, tc_cnn_t * c, const float in_mean[5], con
   if(!c){
     if(s->closure) s->closure->owner = NULL;
     s->closure = NULL;
+    s->gen++;
     return 0;
   }
   if(c->owner && c->owner != s)
@@ -322,6 +331,7 @@ int tc_closure_attach(tc_solver_t * s, tc_cnn_t * c, const float in_mean[5], con
   if(s->closure && s->closure != c) s->closure->owner = NULL;
   s->closure = c;
   c->owner = s;
+  s->gen++;
   return 0;
 }
 
@@ -343,4 +353,5 @@ int tc_closure_eval(tc_solver_t * s){
 void tc_closure_release(tc_solver_t * s){
   if(s->closure) s->closure->owner = NULL;
   s->closure = NULL;
+  s->gen++;
 }

@@ -10,6 +10,12 @@
  * graphs) and replayed by step parity; the members' host-side bookkeeping advances with each
  * replayed step exactly as an eager step would advance it.
  *
+ * The graphs hold the members' buffers, launch shapes, closure network and parameters as they
+ * were when recorded.  Every member carries a generation counter that any change the graphs
+ * cannot see bumps (tc_internal.h); a step finds a member whose counter moved since the
+ * recording, drops the graphs and records again, so replaced fields, a closure coupled,
+ * uncoupled or destroyed, new shapes or a member stepped on its own are always taken up.
+ *
  * Launch shapes: tc_ensemble_autotune times the predictor of every member over the candidate
  * tiles and keeps the fastest, per member (members of the same shape reuse the first result).
  */
@@ -32,6 +38,7 @@ struct tc_ensemble_s {
   int device;
   int recorded;
   long long steps;            /* steps taken since the recording (parity of the replay) */
+  unsigned long long * gen;   /* members' generation counters when the graphs were recorded */
   hipGraph_t graph[2][3];
   hipGraphExec_t exec[2][3];
 };
@@ -43,10 +50,12 @@ int tc_ensemble_create(int n, const tc_params_t * params, const double * const *
   tc_ensemble_t * e = calloc(1, sizeof(*e));
   if(!e) return tc_fail("host allocation failed");
   e->s = calloc((size_t)n, sizeof(tc_solver_t *));
-  if(!e->s){ free(e); return tc_fail("host allocation failed"); }
+  e->gen = calloc((size_t)n, sizeof(unsigned long long));
+  if(!e->s || !e->gen){ free(e->s); free(e->gen); free(e); return tc_fail("host allocation failed"); }
   e->device = params[0].device;
   if(hipSetDevice(e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess){
     free(e->s);
+    free(e->gen);
     free(e);
     return tc_fail("tc_ensemble_create: stream creation failed");
   }
@@ -111,6 +120,7 @@ int tc_ensemble_destroy(tc_ensemble_t * e){
   if(e->fork) hipEventDestroy(e->fork);
   if(e->stream) hipStreamDestroy(e->stream);
   free(e->s);
+  free(e->gen);
   free(e);
   return 0;
 }
@@ -141,40 +151,79 @@ int tc_ensemble_autotune(tc_ensemble_t * e, int reps, int * kb, int * ty){
   return 0;
 }
 
+/* host-side state of a member that enqueueing a substep changes (buffer parity, CFL state):
+ * saved before a recording so that a failed capture, which ran nothing, leaves the member's
+ * bookkeeping matching its buffers */
+typedef struct { int cur[TC_NFIELD]; int dt_valid; } member_state_t;
+
 /* one eager step (so that every member's CFL state is the steady one), then the substeps of
  * two steps recorded; nothing runs while recording, and the members' host state returns to
  * where it was after the two recorded steps (even parity) */
 static int record(tc_ensemble_t * e){
   if(sync_all(e)) return 1;
+  drop_graphs(e);
   for(int m = 0; m < e->n; m++){
     if(tc_enqueue_step(e->s[m])) return 1;
   }
   if(sync_all(e)) return 1;
-  for(int p = 0; p < 2; p++){
-    for(int r = 0; r < 3; r++){
+  member_state_t * saved = malloc(sizeof(member_state_t) * (size_t)e->n);
+  if(!saved) return tc_fail("host allocation failed");
+  for(int m = 0; m < e->n; m++){
+    memcpy(saved[m].cur, e->s[m]->cur, sizeof(saved[m].cur));
+    saved[m].dt_valid = e->s[m]->dt_valid;
+  }
+  int rc = 0;
+  for(int p = 0; p < 2 && !rc; p++){
+    for(int r = 0; r < 3 && !rc; r++){
       /* the lanes fork from the root stream and join it again: the members' kernels form
        * independent branches of the graph */
-      HIPOK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      int rc = hipEventRecord(e->fork, e->stream) != hipSuccess;
-      for(int l = 0; l < e->nlanes && !rc; l++) rc = hipStreamWaitEvent(e->lane[l], e->fork, 0) != hipSuccess;
-      for(int m = 0; m < e->n && !rc; m++) rc = tc_enqueue_substep(e->s[m], r);
-      for(int l = 0; l < e->nlanes && !rc; l++){
-        rc = hipEventRecord(e->join[l], e->lane[l]) != hipSuccess ||
-             hipStreamWaitEvent(e->stream, e->join[l], 0) != hipSuccess;
+      hipError_t ce = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+      if(ce != hipSuccess){
+        rc = tc_fail("stream capture could not begin: %s", hipGetErrorString(ce));
+        break;
+      }
+      int bad = hipEventRecord(e->fork, e->stream) != hipSuccess;
+      for(int l = 0; l < e->nlanes && !bad; l++) bad = hipStreamWaitEvent(e->lane[l], e->fork, 0) != hipSuccess;
+      for(int m = 0; m < e->n && !bad; m++) bad = tc_enqueue_substep(e->s[m], r);
+      for(int l = 0; l < e->nlanes && !bad; l++){
+        bad = hipEventRecord(e->join[l], e->lane[l]) != hipSuccess ||
+              hipStreamWaitEvent(e->stream, e->join[l], 0) != hipSuccess;
       }
       hipGraph_t g = NULL;
-      const hipError_t ce = hipStreamEndCapture(e->stream, &g);
-      if(rc || ce != hipSuccess){
+      ce = hipStreamEndCapture(e->stream, &g);
+      if(bad || ce != hipSuccess){
         if(g) hipGraphDestroy(g);
-        drop_graphs(e);
-        return rc ? 1 : tc_fail("stream capture failed: %s", hipGetErrorString(ce));
+        rc = bad ? 1 : tc_fail("stream capture failed: %s", hipGetErrorString(ce));
+        break;
       }
       e->graph[p][r] = g;
-      HIPOK(hipGraphInstantiate(&e->exec[p][r], g, NULL, NULL, 0));
+      ce = hipGraphInstantiate(&e->exec[p][r], g, NULL, NULL, 0);
+      if(ce != hipSuccess){
+        e->exec[p][r] = NULL;
+        rc = tc_fail("graph instantiation failed: %s", hipGetErrorString(ce));
+      }
     }
   }
+  if(rc){
+    /* nothing recorded ran: the members' bookkeeping goes back to the eager step's end */
+    for(int m = 0; m < e->n; m++){
+      memcpy(e->s[m]->cur, saved[m].cur, sizeof(saved[m].cur));
+      e->s[m]->dt_valid = saved[m].dt_valid;
+    }
+    free(saved);
+    drop_graphs(e);
+    return 1;
+  }
+  free(saved);
+  for(int m = 0; m < e->n; m++) e->gen[m] = e->s[m]->gen;
   e->recorded = 1;
   e->steps = 0;
+  return 0;
+}
+
+/* a member changed since the recording in a way the graphs cannot see */
+static int stale(const tc_ensemble_t * e){
+  for(int m = 0; m < e->n; m++) if(e->s[m]->gen != e->gen[m]) return 1;
   return 0;
 }
 
@@ -182,6 +231,10 @@ int tc_ensemble_step(tc_ensemble_t * e, int nsteps){
   HIPOK(hipSetDevice(e->device));
   if(nsteps < 1) return 0;
   int left = nsteps;
+  if(e->recorded && stale(e)){
+    if(sync_all(e)) return 1;
+    drop_graphs(e);
+  }
   if(!e->recorded){
     for(int m = 0; m < e->n; m++){
       if(e->s[m]->prof_on || e->s[m]->diag_armed || e->s[m]->stat_armed)
@@ -213,8 +266,8 @@ int tc_ensemble_synchronize(tc_ensemble_t * e){
   return sync_all(e);
 }
 
-/* a member's fields were replaced from the host: its recorded step is still valid (same
- * buffers), but the parity of the replay must match the member's buffers again */
+/* drop the recorded graphs; the next step records again.  Changes made through the library
+ * (fields, closure, shapes, solo steps) are detected without it */
 int tc_ensemble_invalidate(tc_ensemble_t * e){
   HIPOK(hipSetDevice(e->device));
   if(sync_all(e)) return 1;

@@ -90,6 +90,10 @@ struct tc_solver_s {
   tc_prof_t prof[TC_NSTAGE];
   int tune[TC_NTUNE][2];
   int predict_window;     /* 1: the predictor variant with the fields' planes staged in LDS */
+  /* bumped by every host-side change that work recorded earlier (an ensemble's graphs)
+   * cannot see: fields replaced, a closure coupled, uncoupled or destroyed, launch shapes,
+   * dt parameters, diagnostics or profiling armed, and every launch outside a replay */
+  unsigned long long gen;
   /* learned closure (tc_cnn.c): network, corrections in the field layout (float), inputs */
   tc_cnn_t * closure;
   void * d_corr[TC_NFIELD];

@@ -63,7 +63,7 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
     const void * twN, int mixed, hipStream_t stream);
 
 /* tck_diag.hip */
-int tck_launch_copy(const double * src, double * dst, long long n, hipStream_t stream);
+int tck_launch_copy(const double * src, double * dst, long long n, int chunks, hipStream_t stream);
 int tck_launch_bc(const tc_geom_t * g, double * a, int kind, double wm, double wp, hipStream_t stream);
 long long tck_diag_cells_blocks(const tc_geom_t * g);
 long long tck_diag_walls_blocks(const tc_geom_t * g);

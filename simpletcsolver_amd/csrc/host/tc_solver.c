@@ -601,6 +601,7 @@ int tc_finish_field(tc_solver_t * s, int f){
   HIPOK(hipStreamSynchronize(s->stream));
   s->dt_valid = 0;
   s->diag_valid = 0;
+  s->gen++;
   return 0;
 }
 
@@ -646,6 +647,7 @@ int tc_set_dt_params(tc_solver_t * s, double coef_dt_adv, double coef_dt_dif, do
   }
   s->dif_limit = lim;
   s->dt_valid = 0;
+  s->gen++;
   return 0;
 }
 
@@ -657,6 +659,7 @@ static const char * g_stage_names[TC_NSTAGE] = {"dt", "factor", "predict", "xsol
 
 int tc_profile_enable(tc_solver_t * s, int on){
   s->prof_on = on;
+  s->gen++;
   if(on){
     for(int n = 0; n < TC_NSTAGE; n++){
       s->prof[n].nrec = 0;
@@ -878,6 +881,7 @@ static int correct(tc_solver_t * s, int rkstep, int mode, int want_cfl){
 }
 
 int tc_enqueue_substep(tc_solver_t * s, int r){
+  s->gen++;
   if(r == 0){
     if(launch_dt(s)) return 1;
     if(factor_tables(s, 0, 2)) return 1;
@@ -973,6 +977,7 @@ int tc_autotune(tc_solver_t * s, int reps, int * kb_out, int * ty_out, double * 
   if(rc) return tc_fail("tc_autotune: launch or timing failed");
   s->tune[TC_TUNE_PREDICT][0] = bkb; s->tune[TC_TUNE_PREDICT][1] = bty;
   s->tune[TC_TUNE_CORRECT][0] = bkb; s->tune[TC_TUNE_CORRECT][1] = bty;
+  s->gen++;
   if(kb_out) *kb_out = bkb;
   if(ty_out) *ty_out = bty;
   if(best_ms) *best_ms = best;
@@ -1021,6 +1026,7 @@ int tc_get_dt(tc_solver_t * s, double * dt){
 
 int tc_decide_dt(tc_solver_t * s, double * dt){
   HIPOK(hipSetDevice(s->p.device));
+  s->gen++;
   s->dt_valid = 0;
   if(launch_dt(s)) return 1;
   return tc_get_dt(s, dt);
@@ -1035,6 +1041,7 @@ static int set_dt(tc_solver_t * s, double dt){
 int tc_predict(tc_solver_t * s, int rkstep, double dt){
   if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
   HIPOK(hipSetDevice(s->p.device));
+  s->gen++;
   if(set_dt(s, dt) || factor_tables(s, rkstep, rkstep) || predict(s, rkstep)) return 1;
   s->dt_valid = 0;
   return tc_synchronize(s);
@@ -1043,6 +1050,7 @@ int tc_predict(tc_solver_t * s, int rkstep, double dt){
 int tc_potential(tc_solver_t * s, int rkstep, double dt){
   if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
   HIPOK(hipSetDevice(s->p.device));
+  s->gen++;
   if(set_dt(s, dt) || potential(s, rkstep)) return 1;
   return tc_synchronize(s);
 }
@@ -1050,6 +1058,7 @@ int tc_potential(tc_solver_t * s, int rkstep, double dt){
 int tc_correct(tc_solver_t * s, int rkstep, double dt){
   if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
   HIPOK(hipSetDevice(s->p.device));
+  s->gen++;
   if(set_dt(s, dt) || correct(s, rkstep, 1, 0)) return 1;
   s->dt_valid = 0;
   return tc_synchronize(s);
@@ -1058,6 +1067,7 @@ int tc_correct(tc_solver_t * s, int rkstep, double dt){
 int tc_pressure(tc_solver_t * s, int rkstep, double dt){
   if(rkstep < 0 || rkstep > 2) return tc_fail("rkstep %d out of range", rkstep);
   HIPOK(hipSetDevice(s->p.device));
+  s->gen++;
   if(set_dt(s, dt) || correct(s, rkstep, 2, 0)) return 1;
   return tc_synchronize(s);
 }
@@ -1127,6 +1137,7 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
   if(kernel && strcmp(kernel, "predict_window") == 0){
     s->predict_window = kb ? 1 : 0;
+    s->gen++;
     return 0;
   }
   int n = -1;
@@ -1136,11 +1147,13 @@ int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
   if(kb < 1 || kb > s->p.nz || !(ty == 1 || ty == 2 || ty == 4)) return tc_fail("tc_tune_set: kb %d, ty %d out of range", kb, ty);
   s->tune[n][0] = kb;
   s->tune[n][1] = ty;
+  s->gen++;
   return 0;
 }
 
 int tc_arm_diagnostics(tc_solver_t * s, int on){
   s->diag_armed = on;
+  s->gen++;
   return 0;
 }
 
@@ -1164,6 +1177,7 @@ int tc_stat_arm(tc_solver_t * s, int on){
     s->stat_num = 0;
   }
   s->stat_armed = on;
+  s->gen++;
   return 0;
 }
 
@@ -1202,8 +1216,11 @@ int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num)
 
 /* measured HBM ceiling: a device copy of nbytes, iters times, timed with events on the
  * solver's stream; bytes moved = 2 nbytes per copy (read + write) */
+/* the copy ceiling: dst = src over nbytes, the better of 4 and 8 independent 16 B chunks per
+ * lane (tck_copy), iters copies each after one untimed one; read + write bytes per second */
 int tc_probe_copy_bandwidth(tc_solver_t * s, size_t nbytes, int iters, double * gbps){
   HIPOK(hipSetDevice(s->p.device));
+  if(iters < 1 || nbytes < 4096) return tc_fail("tc_probe_copy_bandwidth: nbytes %zu, iters %d", nbytes, iters);
   double * a = NULL;
   double * b = NULL;
   HIPOK(hipMalloc((void **)&a, nbytes));
@@ -1211,19 +1228,26 @@ int tc_probe_copy_bandwidth(tc_solver_t * s, size_t nbytes, int iters, double * 
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
-  const long long n = (long long)(nbytes / sizeof(double));
-  int rc = tck_launch_copy(a, b, n, s->stream);
-  hipEventRecord(e0, s->stream);
-  for(int it = 0; it < iters && !rc; it++) rc = tck_launch_copy(it & 1 ? b : a, it & 1 ? a : b, n, s->stream);
-  hipEventRecord(e1, s->stream);
-  hipEventSynchronize(e1);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
+  const long long n = (long long)(nbytes / sizeof(double)) & ~1LL;
+  int rc = 0;
+  double best = 0.;
+  static const int chunks[2] = {4, 8};
+  for(int c = 0; c < 2 && !rc; c++){
+    rc = tck_launch_copy(a, b, n, chunks[c], s->stream);
+    hipEventRecord(e0, s->stream);
+    for(int it = 0; it < iters && !rc; it++) rc = tck_launch_copy(it & 1 ? b : a, it & 1 ? a : b, n, chunks[c], s->stream);
+    hipEventRecord(e1, s->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const double v = 2.0 * (double)n * sizeof(double) * iters / (ms * 1e-3) / 1e9;
+    if(!rc && v > best) best = v;
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   hipFree(a);
   hipFree(b);
   if(rc) return tc_fail("copy kernel failed");
-  *gbps = 2.0 * (double)nbytes * iters / (ms * 1e-3) / 1e9;
+  *gbps = best;
   return 0;
 }

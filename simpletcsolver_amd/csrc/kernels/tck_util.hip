@@ -5,8 +5,10 @@
  *                profile + amp (2u - 1) splitmix64 noise + mode_amp X(i) Y(j) Z(k), evaluated
  *                on the device with the same integer hash and the same operand order as
  *                the host generator, so both give identical values.
- *  tck_copy      dst = src over n doubles, 16 B per lane per access: the measured HBM ceiling
- *                and the reference kernel for scaling hardware byte counters.
+ *  tck_copy      dst = src over n doubles: each lane moves U independent 16 B chunks of one
+ *                block tile (U loads in flight before the first store), loaded and stored
+ *                non-temporally; one tile per block, no grid-stride loop.  The measured HBM
+ *                copy ceiling and the reference kernel for scaling hardware byte counters.
  */
 #include "tck_common.h"
 
@@ -46,10 +48,20 @@ __global__ void tck_ic_fill(const IcArgs A){
   A.dst[tck_idx(A.g, A.i0 + q, j + 1, k + 1)] = v;
 }
 
-__global__ void tck_copy(const double2 * __restrict__ src, double2 * __restrict__ dst, long long n2){
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for(long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += stride){
-    dst[t] = src[t];
+template <int U>
+__global__ __launch_bounds__(TCK_BLOCK) void tck_copy(const double2 * __restrict__ src,
+    double2 * __restrict__ dst, long long n2){
+  const long long base = (long long)blockIdx.x * (TCK_BLOCK * U) + threadIdx.x;
+  double2 v[U];
+  #pragma unroll
+  for(int u = 0; u < U; u++){
+    const long long t = base + (long long)u * TCK_BLOCK;
+    if(t < n2) v[u] = tck_ld(&src[t]);
+  }
+  #pragma unroll
+  for(int u = 0; u < U; u++){
+    const long long t = base + (long long)u * TCK_BLOCK;
+    if(t < n2) tck_st(&dst[t], v[u]);
   }
 }
 
@@ -66,11 +78,18 @@ int tck_launch_ic_fill(const tc_geom_t * g, double * dst, const double * prof, c
   return 0;
 }
 
-int tck_launch_copy(const double * src, double * dst, long long n, hipStream_t stream){
+/* n doubles (even); chunks: 16 B chunks per lane, 4 or 8 */
+int tck_launch_copy(const double * src, double * dst, long long n, int chunks, hipStream_t stream){
   const long long n2 = n / 2;
-  const int nblocks = 256 * 8 * 4;
-  hipLaunchKernelGGL(tck_copy, dim3(nblocks), dim3(TCK_BLOCK), 0, stream,
-      (const double2 *)src, (double2 *)dst, n2);
+  if(chunks == 4){
+    const long long nb = (n2 + TCK_BLOCK * 4 - 1) / (TCK_BLOCK * 4);
+    hipLaunchKernelGGL(tck_copy<4>, dim3((unsigned)nb), dim3(TCK_BLOCK), 0, stream,
+        (const double2 *)src, (double2 *)dst, n2);
+  } else {
+    const long long nb = (n2 + TCK_BLOCK * 8 - 1) / (TCK_BLOCK * 8);
+    hipLaunchKernelGGL(tck_copy<8>, dim3((unsigned)nb), dim3(TCK_BLOCK), 0, stream,
+        (const double2 *)src, (double2 *)dst, n2);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }

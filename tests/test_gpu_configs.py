@@ -4,7 +4,8 @@
   seeded noise of amplitude 1e-2, fixed dt = 1e-3: after 10 steps and after 100;
 * 256 x 512 x 512 Taylor-Couette, Re = 5000, implicit radial diffusion, passive scalar with
   T = 1 and 0 at the walls starting from the conduction profile, noise 1e-2, CFL dt with
-  coefficients 1: after 2 steps (the oracle's share of the run, about 20 s on the host).
+  coefficients 1: after 10 steps (the oracle's share of the run, about a minute on the
+  host's 16 threads).
 
 Radius ratio 0.714 and a tanh-clustered radial grid in both: stretch 2 for the large case,
 stretch 1 for the 64-cell one, whose explicit diffusion at dt = 1e-3 needs its smallest
@@ -12,8 +13,8 @@ radial cell above about 2.4e-3 (with stretch 2 the wall cells are 9.7e-4 wide, t
 limit 1.6e-4, and both codes blow up alike; with stretch 1 they are 3.5e-3 wide); the stretch
 is not part of either configuration, nor are the theta and z lengths, which take the harness
 defaults.  Bar: each field within 1e-10 relative L2 (p and psi modulo their J-weighted mean,
-psi within 1e-8 as in the ten-step tests; after 100 steps psi's difference within 1e-10 of
-the norm of p), equal dt, and max |div u| <= 1e-12.
+psi's difference measured against the norm of p, the field it updates, as in the ten-step
+tests), equal dt, and max |div u| <= 1e-12.
 """
 import numpy as np
 import pytest
@@ -53,13 +54,13 @@ def test_config_64cubed_explicit_100_steps(need_gpu):
     s.close()
 
 
-def test_config_256x512x512_implicit_scalar_2_steps(need_gpu):
+def test_config_256x512x512_implicit_scalar_10_steps(need_gpu):
     case = make_case(nx=256, ny=512, nz=512, Re=5000.0, Pr=1.0, implicit=(1, 0, 0), scalar=1,
                      amp=1e-2, t_walls=(1.0, 0.0))
     o = oracle_for(case)
     s = gpu_for(case)
     del case["fields"]
-    for n in range(2):
+    for n in range(10):
         dto = o.integrate()
         dts = s.step()
         assert abs(dts - dto) <= 1e-15 * abs(dto), (n, dts, dto)

