@@ -179,6 +179,16 @@ int tc_probe_copy_bandwidth(tc_solver_t * s, size_t nbytes, int iters, double * 
  * tanh-clustered grid (stretch s, s = 0 uniform) and laminar Couette + uniform noise */
 int tc_ic_grid(int nx, double lx, int is_curved, double stretch, double * xf, double * xc);
 int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp);
+/* no reference counterpart: tc_ic_fields for callers that cannot allocate device memory
+ * (serving loops that reseed flows, harnesses that forbid hipMalloc).  The same fields, bit
+ * for bit, with no allocation: tc_ic_fields stages its per-index tables in a device buffer it
+ * allocates and frees on every call; this variant stages them in a device array the library
+ * holds for the whole process (4 tables of TC_IC_NOALLOC_N = 4096 doubles, 128 KiB per
+ * device) and its host work space on the stack (64 KiB).  Grids with nx + 2, ny or nz above
+ * 4096 are refused with an error.  All calls on one device share that array: calls that
+ * overlap on one device (threads, or solvers on different streams) race and must be
+ * serialised by the caller; a call leaves the array holding its own tables. */
+int tc_ic_fields_noalloc(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp);
 
 #ifdef __cplusplus
 }

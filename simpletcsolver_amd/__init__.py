@@ -23,6 +23,7 @@ EXPORTED = (
     "tc_decide_dt", "tc_predict", "tc_potential", "tc_correct", "tc_pressure",
     "tc_divergence", "tc_injection", "tc_dissipation", "tc_total_energy",
     "tc_profile_enable", "tc_profile_read", "tc_device_bytes", "tc_ic_grid", "tc_ic_fields",
+    "tc_ic_fields_noalloc",
     "tc_arm_diagnostics", "tc_probe_copy_bandwidth", "tc_set_field_ref", "tc_get_field_ref",
     "tc_set_dt_params", "tc_stat_arm", "tc_stat_collect", "tc_stat_reset", "tc_stat_get",
     "tc_tune_set", "tc_autotune", "tc_ensemble_create", "tc_ensemble_member",
@@ -99,6 +100,7 @@ def load_library(path=None):
     lib.tc_ic_grid.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, dp, dp]
     lib.tc_ic_fields.argtypes = [vp, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_int, ctypes.c_double]
+    lib.tc_ic_fields_noalloc.argtypes = lib.tc_ic_fields.argtypes
     lib.tc_set_field_ref.argtypes = [vp, ctypes.c_char_p, dp]
     lib.tc_get_field_ref.argtypes = [vp, ctypes.c_char_p, dp]
     lib.tc_set_dt_params.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]

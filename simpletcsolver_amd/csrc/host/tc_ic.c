@@ -22,6 +22,7 @@
 #include <hip/hip_runtime_api.h>
 #include "tcstep.h"
 #include "tc_internal.h"
+#include "tc_kernels.h"
 
 static unsigned long long splitmix64(unsigned long long z){
   z += 0x9E3779B97F4A7C15ULL;
@@ -86,29 +87,29 @@ int tck_launch_ic_fill(const tc_geom_t * g, double * dst, const double * prof, c
     const double * my, const double * mz, double amp, double mode_amp, unsigned long long seed,
     unsigned field, int nxs, int i0, hipStream_t stream);
 
-/* The per-index tables (profile, mode factors) are evaluated here; the device kernel
- * tck_ic_fill combines them with the splitmix64 noise in the same operand order as the
- * loop below would, so the field equals the one a host-side generator writes. */
-int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp){
+/* one host table into device staging; the host table is reused as soon as this returns */
+static int put_table(tc_solver_t * s, double * d, const double * h, int n){
+  if(hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+     hipStreamSynchronize(s->stream) != hipSuccess)
+    return tc_fail("initial condition: table upload failed");
+  return 0;
+}
+
+/* The fields of tc_ic_fields, given device staging d (four tables ld doubles apart: profile,
+ * x, y and z mode factors) and host work space tab and pos (nmax doubles each).  The per-index
+ * tables are evaluated here; the device kernel tck_ic_fill combines them with the splitmix64
+ * noise in the same operand order as a host loop would, so the field equals the one a
+ * host-side generator writes. */
+static int ic_fill(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes,
+    double mode_amp, double * d, size_t ld, double * tab, double * pos){
   const int nx = s->p.nx, ny = s->p.ny, nz = s->p.nz;
   const int curved = s->p.is_curved;
   const double * xf = s->h_xf;   /* XF(i) at [i] */
   const double * xc = s->h_xc;
   const double ri = xf[1], ro = xf[nx + 1];
   const double lx = ro - ri;
-  const int nmax = (nx + 2) > ny ? ((nx + 2) > nz ? (nx + 2) : nz) : (ny > nz ? ny : nz);
-  double * h = calloc((size_t)4 * nmax, sizeof(double));
-  double * pos = calloc((size_t)nmax, sizeof(double));
-  double * d = NULL;
-  if(!h || !pos){ free(h); free(pos); return tc_fail("host allocation failed"); }
-  if(hipMalloc((void **)&d, sizeof(double) * 4 * nmax) != hipSuccess){
-    free(h); free(pos);
-    return tc_fail("hipMalloc failed");
-  }
-  double * prof = h, * MX = h + nmax, * Y = h + 2 * nmax, * Z = h + 3 * nmax;
-  int rc = 0;
   static const int fmap[5] = {TC_FUX, TC_FUY, TC_FUZ, TC_FT, TC_FP};
-  for(int f = 0; f < 5 && !rc; f++){
+  for(int f = 0; f < 5; f++){
     const int nxs = f == 0 ? nx + 1 : nx + 2;
     const int i0 = f == 0 ? 1 : 0;
     for(int q = 0; q < nxs; q++){
@@ -116,30 +117,64 @@ int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_
       double v = 0.;
       if(f == 1) v = laminar_uy(r, ri, ro, s->p.uy_xm, s->p.uy_xp, curved);
       if(f == 3) v = conduction_t(r, ri, ro, s->p.t_xm, s->p.t_xp, curved);
-      prof[q] = v;
+      tab[q] = v;
     }
+    if(put_table(s, d, tab, nxs)) return 1;
     const int use_modes = nmodes > 0 && mode_amp != 0. && f < 3;
     if(use_modes){
       for(int q = 0; q < nxs; q++) pos[q] = ((f == 0 ? xf[q + 1] : xc[q]) - ri) / lx;
-      mode_table(seed, f, 0, nmodes, 0, nxs, pos, MX);
+      mode_table(seed, f, 0, nmodes, 0, nxs, pos, tab);
+      if(put_table(s, d + ld, tab, nxs)) return 1;
       for(int q = 0; q < ny; q++) pos[q] = (double)q / ny;
-      mode_table(seed, f, 1, nmodes, 1, ny, pos, Y);
+      mode_table(seed, f, 1, nmodes, 1, ny, pos, tab);
+      if(put_table(s, d + 2 * ld, tab, ny)) return 1;
       for(int q = 0; q < nz; q++) pos[q] = (double)q / nz;
-      mode_table(seed, f, 2, nmodes, 1, nz, pos, Z);
+      mode_table(seed, f, 2, nmodes, 1, nz, pos, tab);
+      if(put_table(s, d + 3 * ld, tab, nz)) return 1;
     }
     const double a = f < 3 ? amp : (f == 3 ? t_amp : 0.);
     double * dst = tc_field_ptr(s, fmap[f]);
-    if(hipMemcpyAsync(d, h, sizeof(double) * 4 * nmax, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
-       tck_launch_ic_fill(&s->g, dst, d, use_modes ? d + nmax : NULL, d + 2 * nmax, d + 3 * nmax, a,
+    if(tck_launch_ic_fill(&s->g, dst, d, use_modes ? d + ld : NULL, d + 2 * ld, d + 3 * ld, a,
          mode_amp, seed, (unsigned)f, nxs, i0, s->stream) != 0 ||
-       hipStreamSynchronize(s->stream) != hipSuccess){
-      rc = tc_fail("initial condition fill failed");
-      break;
-    }
-    rc = tc_finish_field(s, fmap[f]);
+       hipStreamSynchronize(s->stream) != hipSuccess)
+      return tc_fail("initial condition fill failed");
+    if(tc_finish_field(s, fmap[f])) return 1;
   }
+  return 0;
+}
+
+static int ic_nmax(const tc_solver_t * s){
+  const int a = s->p.nx + 2, b = s->p.ny, c = s->p.nz;
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+int tc_ic_fields(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp){
+  HIPOK(hipSetDevice(s->p.device));
+  const int nmax = ic_nmax(s);
+  double * h = calloc((size_t)2 * nmax, sizeof(double));
+  double * d = NULL;
+  if(!h) return tc_fail("host allocation failed");
+  if(hipMalloc((void **)&d, sizeof(double) * 4 * nmax) != hipSuccess){
+    free(h);
+    return tc_fail("hipMalloc failed");
+  }
+  const int rc = ic_fill(s, seed, amp, t_amp, nmodes, mode_amp, d, (size_t)nmax, h, h + nmax);
   hipFree(d);
   free(h);
-  free(pos);
   return rc;
+}
+
+/* the same fields with no allocation: device staging in the library's per-device array of
+ * four TC_IC_NOALLOC_N-double tables (tck_ic_scratch), host work space on the stack */
+int tc_ic_fields_noalloc(tc_solver_t * s, unsigned long long seed, double amp, double t_amp, int nmodes, double mode_amp){
+  HIPOK(hipSetDevice(s->p.device));
+  const int nmax = ic_nmax(s);
+  if(nmax > TC_IC_NOALLOC_N)
+    return tc_fail("tc_ic_fields_noalloc: grid %d x %d x %d needs tables of %d entries, the limit is %d (nx + 2, ny and nz)",
+        s->p.nx, s->p.ny, s->p.nz, nmax, TC_IC_NOALLOC_N);
+  double * d = NULL;
+  const int e = tck_ic_scratch(&d);
+  if(e) return tc_fail("tc_ic_fields_noalloc: device address of the staging array: %s", hipGetErrorString((hipError_t)e));
+  double tab[TC_IC_NOALLOC_N], pos[TC_IC_NOALLOC_N];
+  return ic_fill(s, seed, amp, t_amp, nmodes, mode_amp, d, TC_IC_NOALLOC_N, tab, pos);
 }

@@ -62,6 +62,10 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
 int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const void * twM,
     const void * twN, int mixed, hipStream_t stream);
 
+/* tck_util.hip: device address of this device's staging array of the allocation-free
+ * initial-condition fill (4 tables of TC_IC_NOALLOC_N doubles), looked up once per device */
+int tck_ic_scratch(double ** ptr);
+
 /* tck_diag.hip */
 int tck_launch_copy(const double * src, double * dst, long long n, int chunks, hipStream_t stream);
 int tck_launch_bc(const tc_geom_t * g, double * a, int kind, double wm, double wp, hipStream_t stream);

@@ -11,6 +11,35 @@
  *                copy ceiling and the reference kernel for scaling hardware byte counters.
  */
 #include "tck_common.h"
+#include <mutex>
+
+/* Staging of the allocation-free initial-condition fill (tc_ic_fields_noalloc): the four
+ * per-index tables of one field (profile, x, y and z mode factors), TC_IC_NOALLOC_N doubles
+ * each, in one file-scope device array.  Every device the library runs on holds its own copy
+ * (128 KiB) until the process ends.  Host code never passes the array's name to a kernel or a
+ * copy: tck_ic_scratch looks up its device address once per device (hipGetSymbolAddress with
+ * that device current) and hands that out. */
+static __device__ __align__(256) double scratch[4 * TC_IC_NOALLOC_N];
+
+#define TCK_MAXDEV 64
+static std::mutex g_scratch_mu;
+static double * g_scratch_ptr[TCK_MAXDEV];
+
+extern "C" int tck_ic_scratch(double ** ptr){
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if(e != hipSuccess) return (int)e;
+  if(dev < 0 || dev >= TCK_MAXDEV) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if(!g_scratch_ptr[dev]){
+    void * p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if(e != hipSuccess) return (int)e;
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return 0;
+}
 
 __device__ __forceinline__ unsigned long long tck_splitmix64(unsigned long long z){
   z += 0x9E3779B97F4A7C15ULL;

@@ -18,6 +18,10 @@ extern "C" {
 
 #define TC_XOFF 15
 
+/* largest table length of the allocation-free initial-condition fill: nx + 2, ny and nz
+ * may not exceed it (tc_ic_fields_noalloc) */
+#define TC_IC_NOALLOC_N 4096
+
 typedef struct {
   int nx, ny, nz;
   int px, py, pz;
