@@ -1214,10 +1214,8 @@ int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num)
   return 0;
 }
 
-/* measured HBM ceiling: a device copy of nbytes, iters times, timed with events on the
- * solver's stream; bytes moved = 2 nbytes per copy (read + write) */
-/* the copy ceiling: dst = src over nbytes, the better of 4 and 8 independent 16 B chunks per
- * lane (tck_copy), iters copies each after one untimed one; read + write bytes per second */
+/* the copy ceiling: dst = src over nbytes, the best of 1, 2, 4 and 8 independent 16 B chunks
+ * per lane (tck_copy), iters copies each after one untimed one; read + write bytes per second */
 int tc_probe_copy_bandwidth(tc_solver_t * s, size_t nbytes, int iters, double * gbps){
   HIPOK(hipSetDevice(s->p.device));
   if(iters < 1 || nbytes < 4096) return tc_fail("tc_probe_copy_bandwidth: nbytes %zu, iters %d", nbytes, iters);
@@ -1231,8 +1229,8 @@ int tc_probe_copy_bandwidth(tc_solver_t * s, size_t nbytes, int iters, double * 
   const long long n = (long long)(nbytes / sizeof(double)) & ~1LL;
   int rc = 0;
   double best = 0.;
-  static const int chunks[2] = {4, 8};
-  for(int c = 0; c < 2 && !rc; c++){
+  static const int chunks[4] = {1, 2, 4, 8};
+  for(int c = 0; c < 4 && !rc; c++){
     rc = tck_launch_copy(a, b, n, chunks[c], s->stream);
     hipEventRecord(e0, s->stream);
     for(int it = 0; it < iters && !rc; it++) rc = tck_launch_copy(it & 1 ? b : a, it & 1 ? a : b, n, chunks[c], s->stream);

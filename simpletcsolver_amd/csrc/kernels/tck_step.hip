@@ -586,67 +586,77 @@ struct CorrectArgs {
   TileMap tm;
 };
 
-template <typename R>
-__device__ __forceinline__ double correct_body(const CorrectArgs & A, double * __restrict__ ux,
-    double * __restrict__ uy, double * __restrict__ uz, double * __restrict__ pp,
-    const R * __restrict__ psi, const int i, const int j, const int k, const long long kb0){
-  const tc_geom_t & g = A.g;
+/* Per-column constants of the correction: everything in correct/*.c and update_pressure.c
+ * that depends on i and the substep only (the reference's left-to-right evaluation of
+ * prefactor / HXXF(i) * (...) and 2 f / J * (...) makes these factors exact subexpressions),
+ * evaluated once per thread before its plane loop. */
+struct CorrectCol {
+  double fx, fy, fz;        /* gamma dt / HXXF(i), gamma dt / HYXC(i), gamma dt / hz */
+  double hx, hy;            /* HXXF(i), HYXC(i) (CFL bounds) */
+  double qf;                /* 2 f / JDXC(i), f = gamma dt nu / 2 */
+  double xl, xc, xu, yl, yc, zl, zc;   /* implicit-direction Laplacian of psi */
+};
+
+__device__ __forceinline__ CorrectCol correct_col(const CorrectArgs & A, int i){
   const tc_coef_t & c = A.c;
-  const bool inside = (i <= g.nx && j <= g.ny);
+  CorrectCol C;
+  const double dt = *A.dt;
+  const int s = A.rkstep;
+  const double gamma = c_rka[s] + c_rkb[s];
+  const double prefactor = gamma * dt;                /* correct/main.c:22-23 */
+  C.hx = c.hxxf[i];
+  C.hy = c.hyxc[i];
+  C.fx = prefactor / C.hx;
+  C.fy = prefactor / C.hy;
+  C.fz = prefactor / c.hz;
+  const double pf = 0.5 * gamma * dt * c.nu;          /* update_pressure.c:155-156 */
+  C.qf = 2. * pf / c.jdxc[i];
+  C.xl = c.plx_l[i]; C.xu = c.plx_u[i]; C.xc = - C.xl - C.xu;
+  C.yl = c.ply[i]; C.yc = - C.yl - C.yl;
+  C.zl = c.plz[i]; C.zc = - C.zl - C.zl;
+  return C;
+}
+
+/* one cell of the correction; pointers start at plane k-1 (kb0), 32-bit offsets inside the
+ * 3-plane window.  CFL: also the cell's bound of decide_dt_adv on the corrected velocity. */
+template <typename R, bool CFL>
+__device__ __forceinline__ double correct_body(const CorrectArgs & A, const CorrectCol & C,
+    double * __restrict__ ux, double * __restrict__ uy, double * __restrict__ uz,
+    double * __restrict__ pp, const R * __restrict__ psi, const int i, const int j, const int k,
+    const long long kb0){
+  const tc_geom_t & g = A.g;
   double lmin = 1.;
-  if(inside){
-    /* pointers start at plane k-1 (kb0); 32-bit offsets inside the 3-plane window */
-    const int sy = g.px;
-    const int sz = (int)g.plane;
-    const int o = (i + TC_XOFF) + sy * j + sz;
-    const double dt = *A.dt;
-    const int s = A.rkstep;
-    const double gamma = c_rka[s] + c_rkb[s];
-    const double prefactor = gamma * dt;                /* correct/main.c:22-23 */
-    const double small = 1.e-8;
-    const double ps0 = double(psi[o]);
-    if(A.mode & 1){
+  const int sy = g.px;
+  const int sz = (int)g.plane;
+  const int o = (i + TC_XOFF) + sy * j + sz;
+  const double small = 1.e-8;
+  const double ps0 = double(psi[o]);
+  if(A.mode & 1){
     if(i >= 2){ /* correct/ux.c:25-52 */
-      const double hx = c.hxxf[i];
-      const double v = ux[o] - prefactor / hx * (- double(psi[o - 1]) + ps0);
+      const double v = ux[o] - C.fx * (- double(psi[o - 1]) + ps0);
       tck_store_halo(ux - kb0, g, i, j, k, v, false);
-      lmin = fmin(lmin, hx / (fabs(v) + small));
+      if(CFL) lmin = fmin(lmin, C.hx / (fabs(v) + small));
     }
     { /* correct/uy.c:25-52 */
-      const double hy = c.hyxc[i];
-      const double v = uy[o] - prefactor / hy * (- double(psi[o - sy]) + ps0);
+      const double v = uy[o] - C.fy * (- double(psi[o - sy]) + ps0);
       tck_store_halo(uy - kb0, g, i, j, k, v, false);
-      lmin = fmin(lmin, hy / (fabs(v) + small));
+      if(CFL) lmin = fmin(lmin, C.hy / (fabs(v) + small));
     }
     { /* correct/uz.c:15-44 */
-      const double v = uz[o] - prefactor / c.hz * (- double(psi[o - sz]) + ps0);
+      const double v = uz[o] - C.fz * (- double(psi[o - sz]) + ps0);
       tck_store_halo(uz - kb0, g, i, j, k, v, false);
-      lmin = fmin(lmin, c.hz / (fabs(v) + small));
+      if(CFL) lmin = fmin(lmin, A.c.hz / (fabs(v) + small));
     }
-    }
-    if(A.mode & 2){ /* update_pressure.c:146-173 */
-      double v = pp[o] + ps0;
-      const double pf = 0.5 * gamma * dt * c.nu;
-      if(A.f.implicit[0]){
-        const double l = c.plx_l[i];
-        const double u = c.plx_u[i];
-        const double cc = - l - u;
-        v -= 2. * pf / c.jdxc[i] * (+ l * double(psi[o - 1]) + cc * ps0 + u * double(psi[o + 1]));
-      }
-      if(A.f.implicit[1]){
-        const double l = c.ply[i];
-        const double u = c.ply[i];
-        const double cc = - l - u;
-        v -= 2. * pf / c.jdxc[i] * (+ l * double(psi[o - sy]) + cc * ps0 + u * double(psi[o + sy]));
-      }
-      if(A.f.implicit[2]){
-        const double l = c.plz[i];
-        const double u = c.plz[i];
-        const double cc = - l - u;
-        v -= 2. * pf / c.jdxc[i] * (+ l * double(psi[o - sz]) + cc * ps0 + u * double(psi[o + sz]));
-      }
-      tck_store_halo(pp - kb0, g, i, j, k, v, true);
-    }
+  }
+  if(A.mode & 2){ /* update_pressure.c:146-173 */
+    double v = pp[o] + ps0;
+    if(A.f.implicit[0])
+      v -= C.qf * (+ C.xl * double(psi[o - 1]) + C.xc * ps0 + C.xu * double(psi[o + 1]));
+    if(A.f.implicit[1])
+      v -= C.qf * (+ C.yl * double(psi[o - sy]) + C.yc * ps0 + C.yl * double(psi[o + sy]));
+    if(A.f.implicit[2])
+      v -= C.qf * (+ C.zl * double(psi[o - sz]) + C.zc * ps0 + C.zl * double(psi[o + sz]));
+    tck_store_halo(pp - kb0, g, i, j, k, v, true);
   }
   return lmin;
 }
@@ -666,7 +676,7 @@ __device__ __forceinline__ void dt_block_min(double v, unsigned long long * dtmi
   }
 }
 
-template <typename R>
+template <typename R, bool CFL>
 __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
   int bx, by, bz;
   tile_of(A.tm, &bx, &by, &bz);
@@ -675,14 +685,16 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
   const int k0 = 1 + bz * A.tm.kb;
   const int k1 = min(k0 + A.tm.kb, A.g.nz + 1);
   double lmin = 1.;
-  for(int k = k0; k < k1; k++){
-    int il = i;
-    asm volatile("" : "+v"(il));
-    const long long kb0 = (long long)(k - 1) * A.g.plane;
-    lmin = fmin(lmin, correct_body<R>(A, A.ux + kb0, A.uy + kb0, A.uz + kb0, A.p + kb0, (const R *)A.psi + kb0, il, j, k, kb0));
+  if(i <= A.g.nx && j <= A.g.ny){
+    const CorrectCol C = correct_col(A, i);
+    for(int k = k0; k < k1; k++){
+      const long long kb0 = (long long)(k - 1) * A.g.plane;
+      lmin = fmin(lmin, correct_body<R, CFL>(A, C, A.ux + kb0, A.uy + kb0, A.uz + kb0, A.p + kb0,
+          (const R *)A.psi + kb0, i, j, k, kb0));
+    }
   }
   /* CFL bound of decide_dt_adv (decide_dt.c:32-80) on the corrected velocity */
-  if(A.dtmin) dt_block_min(lmin, A.dtmin);
+  if(CFL) dt_block_min(lmin, A.dtmin);
 }
 
 /* decide_dt_adv on the current field (decide_dt.c:32-80), for the first step and after
@@ -803,8 +815,14 @@ int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
   A.ux = ux; A.uy = uy; A.uz = uz; A.p = p; A.psi = psi;
   A.dt = dt; A.rkstep = rkstep; A.dtmin = dtmin;
   A.tm = cell_tiles(*g, kb, ty);
-  if(mixed) hipLaunchKernelGGL(tck_correct<float>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
-  else hipLaunchKernelGGL(tck_correct<double>, dim3((unsigned)A.tm.total), dim3(TCK_WAVE, ty), 0, stream, A);
+  const dim3 grid((unsigned)A.tm.total), block(TCK_WAVE, ty);
+  if(mixed){
+    if(dtmin) hipLaunchKernelGGL((tck_correct<float, true>), grid, block, 0, stream, A);
+    else hipLaunchKernelGGL((tck_correct<float, false>), grid, block, 0, stream, A);
+  } else {
+    if(dtmin) hipLaunchKernelGGL((tck_correct<double, true>), grid, block, 0, stream, A);
+    else hipLaunchKernelGGL((tck_correct<double, false>), grid, block, 0, stream, A);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }

@@ -8,7 +8,8 @@
  *  tck_copy      dst = src over n doubles: each lane moves U independent 16 B chunks of one
  *                block tile (U loads in flight before the first store), loaded and stored
  *                non-temporally; one tile per block, no grid-stride loop.  The measured HBM
- *                copy ceiling and the reference kernel for scaling hardware byte counters.
+ *                copy ceiling (the best of U = 1, 2, 4 and 8: on MI355X many small blocks,
+ *                U = 1, copy fastest) and the reference kernel for scaling byte counters.
  */
 #include "tck_common.h"
 #include <mutex>
@@ -94,6 +95,13 @@ __global__ __launch_bounds__(TCK_BLOCK) void tck_copy(const double2 * __restrict
   }
 }
 
+template <int U>
+static void launch_copy(const double * src, double * dst, long long n2, hipStream_t stream){
+  const long long nb = (n2 + TCK_BLOCK * U - 1) / (TCK_BLOCK * U);
+  hipLaunchKernelGGL(tck_copy<U>, dim3((unsigned)nb), dim3(TCK_BLOCK), 0, stream,
+      (const double2 *)src, (double2 *)dst, n2);
+}
+
 extern "C" {
 
 int tck_launch_ic_fill(const tc_geom_t * g, double * dst, const double * prof, const double * mx,
@@ -107,17 +115,15 @@ int tck_launch_ic_fill(const tc_geom_t * g, double * dst, const double * prof, c
   return 0;
 }
 
-/* n doubles (even); chunks: 16 B chunks per lane, 4 or 8 */
+/* n doubles (even); chunks: 16 B chunks per lane, 1, 2, 4 or 8 */
 int tck_launch_copy(const double * src, double * dst, long long n, int chunks, hipStream_t stream){
   const long long n2 = n / 2;
-  if(chunks == 4){
-    const long long nb = (n2 + TCK_BLOCK * 4 - 1) / (TCK_BLOCK * 4);
-    hipLaunchKernelGGL(tck_copy<4>, dim3((unsigned)nb), dim3(TCK_BLOCK), 0, stream,
-        (const double2 *)src, (double2 *)dst, n2);
-  } else {
-    const long long nb = (n2 + TCK_BLOCK * 8 - 1) / (TCK_BLOCK * 8);
-    hipLaunchKernelGGL(tck_copy<8>, dim3((unsigned)nb), dim3(TCK_BLOCK), 0, stream,
-        (const double2 *)src, (double2 *)dst, n2);
+  switch(chunks){
+    case 1: launch_copy<1>(src, dst, n2, stream); break;
+    case 2: launch_copy<2>(src, dst, n2, stream); break;
+    case 4: launch_copy<4>(src, dst, n2, stream); break;
+    case 8: launch_copy<8>(src, dst, n2, stream); break;
+    default: return (int)hipErrorInvalidValue;
   }
   TCK_CHECK_LAUNCH();
   return 0;
