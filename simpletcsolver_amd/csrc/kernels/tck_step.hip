@@ -86,7 +86,7 @@ __device__ __forceinline__ double own_value(const FT * staged, int of, const dou
 
 /* body with restrict-qualified pointers, so that the loads of each field section are free
  * to move ahead of the stores of the previous one */
-template <typename R, typename PT, typename FT, int TW>
+template <typename R, typename PT, typename FT, int TW, int CM>
 __device__ __forceinline__ void predict_body(const PredictArgs & A,
     const FT * __restrict__ Fux_m, const FT * __restrict__ Fux_0, const FT * __restrict__ Fux_p,
     const FT * __restrict__ Fuy_m, const FT * __restrict__ Fuy_0, const FT * __restrict__ Fuy_p,
@@ -98,7 +98,8 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     const R * __restrict__ sbx, const R * __restrict__ sby, const R * __restrict__ sbz,
     const R * __restrict__ sbt, const R * sbv,
     R * __restrict__ sax, R * __restrict__ say, R * __restrict__ saz, R * __restrict__ sat,
-    const int i, const int j, const int k, const long long kb0, const R * smt, const int ib){
+    const int i, const int j, const int k, const long long kb0, const R * smt, const int ib,
+    const R dt, const float * cvv){
   const tc_geom_t & g = A.g;
   const tc_coef_t & c = A.c;
   if(i > g.nx || j > g.ny) return;
@@ -109,7 +110,6 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   const int o = (i + TC_XOFF) + sy * j + sz;
   const R nu = R(c.nu);
   const R nu2 = R(2.) * nu;
-  const R dt = *A.dt;
   const int s = A.rkstep;
   const R ca = c_rka[s] * dt;
   const R cb = c_rkb[s] * dt;
@@ -184,7 +184,8 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     }
     /* pressure :377-394 */
     gg -= CT(T_IHXF, i) * (- R(Fp_0[of - 1]) + R(Fp_0[of]));
-    if(A.cx) a += R(A.cx[kb0 + o]);
+    if(CM > 0) a += R(cvv[0]);
+    else if(CM < 0 && A.cx) a += R(A.cx[kb0 + o]);
     R b = R(0.);
     if(sbx) b = sbv ? sbv[0] : sbx[o];
     const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -272,7 +273,8 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     }
     /* pressure :402-419 */
     gg -= CT(T_IHYC, i) * (- R(Fp_0[of - fsy]) + R(Fp_0[of]));
-    if(A.cy) a += R(A.cy[kb0 + o]);
+    if(CM > 0) a += R(cvv[1]);
+    else if(CM < 0 && A.cy) a += R(A.cy[kb0 + o]);
     R b = R(0.);
     if(sby) b = sbv ? sbv[1] : sby[o];
     const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -340,7 +342,8 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     }
     /* pressure :301-318 */
     gg -= R(c.ihz) * (- R(Fp_m[of]) + R(Fp_0[of]));
-    if(A.cz) a += R(A.cz[kb0 + o]);
+    if(CM > 0) a += R(cvv[2]);
+    else if(CM < 0 && A.cz) a += R(A.cz[kb0 + o]);
     R b = R(0.);
     if(sbz) b = sbv ? sbv[2] : sbz[o];
     const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -389,7 +392,8 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
       const R vz = kappa * (+ R(c.lapz_l) * tzm + R(c.lapz_c) * t0 + R(c.lapz_u) * tzp);
       if(iz) gg += vz; else a += vz;
     }
-    if(A.ct) a += R(A.ct[kb0 + o]);
+    if(CM > 0) a += R(cvv[3]);
+    else if(CM < 0 && A.ct) a += R(A.ct[kb0 + o]);
     R b = R(0.);
     if(sbt) b = sbv ? sbv[3] : sbt[o];
     const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
@@ -422,6 +426,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
   const int k1 = min(k0 + A.tm.kb, A.g.nz + 1);
   const double * tf = A.t ? A.t : A.ux;   /* T is read only with the scalar on */
   const long long pl = A.g.plane;
+  const R dtv = *A.dt;
   for(int k = k0; k < k1; k++){
     const long long kb0 = (long long)(k - 1) * pl;
     /* field offset within a plane; planes k-1, k, k+1 through three base pointers */
@@ -430,7 +435,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
     #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
     #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
-    predict_body<R, double, double, TCK_TW>(A, A.ux + kb0, A.ux + kb0 + pl, A.ux + kb0 + 2 * pl,
+    predict_body<R, double, double, TCK_TW, -1>(A, A.ux + kb0, A.ux + kb0 + pl, A.ux + kb0 + 2 * pl,
         A.uy + kb0, A.uy + kb0 + pl, A.uy + kb0 + 2 * pl,
         A.uz + kb0, A.uz + kb0 + pl, A.uz + kb0 + 2 * pl,
         tf + kb0, tf + kb0 + pl, tf + kb0 + 2 * pl,
@@ -438,7 +443,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
         RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
         RB(A.nx_), RB(A.ny_), RB(A.nz_), RB(A.nt),
         RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), (const R *)nullptr, RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat),
-        i, j, k, kb0, smt, ib);
+        i, j, k, kb0, smt, ib, dtv, (const float *)nullptr);
     #undef RB
     #undef RBT
     #undef RBC
@@ -470,14 +475,17 @@ __device__ __forceinline__ void tck_lds_barrier(){
 #define TCK_WPF ((TCK_WPLANE + TCK_BLOCK - 1) / TCK_BLOCK)   /* window elements per thread */
 
 
-template <typename R>
+/* SRC: RK sources of the previous substep are read (substeps 1 and 2); CM: closure
+ * corrections are added.  Each combination is its own instantiation, so that the registers of
+ * what a launch does not read are not held. */
+template <typename R, bool SRC, bool CM>
 __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) tck_predict_w(const PredictArgs A){
   __shared__ R smt[T_NPRED * TCK_WTW];
-  __shared__ R wf[4][3][TCK_WPLANE];        /* ux uy uz T: a ring of three planes, in the
-                                             * arithmetic type (the stencils read them only
-                                             * through R()) */
-  __shared__ R wq[2][TCK_WPLANE];           /* p: two planes, in the arithmetic type (the
-                                             * stencils read p only through R()) */
+  /* ux uy uz T: a ring of three planes; p: two planes; in the arithmetic type (the stencils
+   * read them only through R()).  Each plane has one spare slot past the window, where the
+   * threads whose second element lies outside the window put it: the puts are unconditional */
+  __shared__ R wf[4][3][TCK_WPLANE + 1];
+  __shared__ R wq[2][TCK_WPLANE + 1];
   int bx, by, bz;
   tile_of(A.tm, &bx, &by, &bz);
   const int ib = bx * TCK_WX;             /* i of LDS column 0: the tile starts at i = ib + 1 */
@@ -498,68 +506,90 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
   const int nf = A.t ? 4 : 3;
   const long long pl = A.g.plane;
   const long long px = A.g.px;
+  /* T's slot reads ux when the scalar is off (loaded, never used) */
   const double * fld[5] = {A.ux, A.uy, A.uz, A.t ? A.t : A.ux, A.p};
-  /* this thread's window elements e = tid + q TCK_BLOCK: field offsets within a plane (-1:
-   * outside the field) */
+  /* this thread's window elements e = tid + q TCK_BLOCK: field offsets within a plane, an
+   * in-bounds stand-in (offset 0) for elements outside the field, whose value is replaced by
+   * 0 when put; LDS slots, the spare one past the window for e beyond it */
   long long woff[TCK_WPF];
+  bool wok[TCK_WPF];
+  int wslot[TCK_WPF];
   #pragma unroll
   for(int q = 0; q < TCK_WPF; q++){
     const int e = tid + q * TCK_BLOCK;
     const int r = e / TCK_WTW, cc = e - r * TCK_WTW;
-    woff[q] = (e < TCK_WPLANE && r < nrow && cc < ncol) ? (long long)(ib + cc + TC_XOFF) + px * (jw + r) : -1;
+    wok[q] = e < TCK_WPLANE && r < nrow && cc < ncol;
+    woff[q] = wok[q] ? (long long)(ib + cc + TC_XOFF) + px * (jw + r) : 0;
+    wslot[q] = e < TCK_WPLANE ? e : TCK_WPLANE;
   }
   /* FP64 until the plane is put in LDS: a conversion here would wait for the loads */
   double pre[5][TCK_WPF];
-  /* the cell's RK sources of the previous substep (substeps 1 and 2): loaded with the next
-   * plane in the mixed step; the FP64 form has no registers to spare for them and the body
-   * loads them where they are used */
-  constexpr bool PFS = sizeof(R) == 4;
+  /* Every global load of the plane loop is unconditional (out-of-range lanes read an
+   * in-bounds stand-in and discard it), and the cell's RK sources of the previous substep
+   * (substeps 1 and 2) and closure corrections are loaded at the top of each plane, after the
+   * staged plane is put in LDS and before the next plane's loads and any store of the plane,
+   * and waited for before the first store.  The vector-memory counter counts loads and stores
+   * together and the compiler can wait for a load after a store only by draining everything,
+   * so this order leaves the next plane's loads in flight through the whole plane; dt is read
+   * once. */
   const bool inside = i <= A.g.nx && j <= A.g.ny;
-  const long long ocell = (long long)(i + TC_XOFF) + px * j;
-  const R * sbp[4] = {(const R *)A.sbx, (const R *)A.sby, (const R *)A.sbz, (const R *)A.sbt};
-  R psb[4] = {R(0.), R(0.), R(0.), R(0.)};
-  /* fields f < nf at plane kk and p at plane kq into registers; sources at plane ks */
-  #define TCK_FETCH(kk, kq, ks) \
+  const long long ocell = (long long)(min(i, A.g.nx) + TC_XOFF) + px * min(j, A.g.ny);
+  const R * sbp[4] = {(const R *)A.sbx, (const R *)A.sby, (const R *)A.sbz,
+                      (const R *)(A.sbt ? A.sbt : A.sbx)};
+  const float * cpp[4] = {A.cx, A.cy, A.cz, A.ct ? A.ct : A.cx};
+  const R dtv = *A.dt;
+  /* fields at plane kk and p at plane kq into registers */
+  #define TCK_FETCH(kk, kq) \
     _Pragma("unroll") for(int f = 0; f < 5; f++) \
       _Pragma("unroll") for(int q = 0; q < TCK_WPF; q++) \
-        pre[f][q] = (woff[q] >= 0 && (f < nf || f == 4)) ? fld[f][woff[q] + pl * (f == 4 ? (kq) : (kk))] : 0.; \
-    if(PFS){ _Pragma("unroll") for(int f = 0; f < 4; f++) \
-      psb[f] = (inside && sbp[f]) ? sbp[f][ocell + pl * (ks)] : R(0.); }
+        pre[f][q] = fld[f][woff[q] + pl * (f == 4 ? (kq) : (kk))];
   #define TCK_PUT(slot, pslot) \
     _Pragma("unroll") for(int q = 0; q < TCK_WPF; q++){ \
-      const int e = tid + q * TCK_BLOCK; \
-      if(e < TCK_WPLANE){ \
-        _Pragma("unroll") for(int f = 0; f < 4; f++) wf[f][slot][e] = R(pre[f][q]); \
-        wq[pslot][e] = R(pre[4][q]); \
-      } \
+      _Pragma("unroll") for(int f = 0; f < 4; f++) \
+        wf[f][slot][wslot[q]] = (wok[q] && f < nf) ? R(pre[f][q]) : R(0.); \
+      wq[pslot][wslot[q]] = wok[q] ? R(pre[4][q]) : R(0.); \
     }
   int sm = 0, s0 = 1, sp = 2, pm = 0;
-  TCK_FETCH(k0 - 1, k0 - 1, k0);
+  TCK_FETCH(k0 - 1, k0 - 1);
   TCK_PUT(sm, pm);
-  TCK_FETCH(k0, k0, k0);
+  TCK_FETCH(k0, k0);
   TCK_PUT(s0, 1 - pm);          /* p's plane k0 is written again below */
-  TCK_FETCH(k0 + 1, k0, k0);
+  TCK_FETCH(k0 + 1, k0);
   const int of = (li + 1) + TCK_WTW * (lj + 1);
   const int ft = nf == 4 ? 3 : 0;
   for(int k = k0; k < k1; k++){
     TCK_PUT(sp, 1 - pm);        /* fields: plane k+1; p: plane k */
-    R sbv[4];
-    #pragma unroll
-    for(int f = 0; f < 4; f++) sbv[f] = psb[f];
+    R sbv[4] = {R(0.), R(0.), R(0.), R(0.)};
+    float cvv[4] = {0.f, 0.f, 0.f, 0.f};
+    if(SRC){
+      #pragma unroll
+      for(int f = 0; f < 4; f++) sbv[f] = sbp[f][ocell + pl * k];
+    }
+    if(CM){
+      #pragma unroll
+      for(int f = 0; f < 4; f++) cvv[f] = cpp[f][ocell + pl * k];
+    }
     tck_lds_barrier();
-    if(k + 1 < k1){ TCK_FETCH(k + 2, k + 1, k + 1); }
+    /* the last plane re-reads plane k + 1 (in bounds) instead of branching around the loads */
+    const int kn = k + 1 < k1 ? k + 2 : k + 1;
+    TCK_FETCH(kn, kn - 1);
+    /* sources and corrections are in registers before the first store */
+    if(SRC){ asm volatile("" :: "v"(sbv[0]), "v"(sbv[1]), "v"(sbv[2]), "v"(sbv[3])); }
+    if(CM){ asm volatile("" :: "v"(cvv[0]), "v"(cvv[1]), "v"(cvv[2]), "v"(cvv[3])); }
 
     const long long kb0 = (long long)(k - 1) * pl;
     #define RB(ptr) ((ptr) ? (ptr) + kb0 : nullptr)
     #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
     #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
-    predict_body<R, R, R, TCK_WTW>(A, wf[0][sm], wf[0][s0], wf[0][sp], wf[1][sm], wf[1][s0], wf[1][sp],
+    if(inside)
+    predict_body<R, R, R, TCK_WTW, CM ? 1 : 0>(A, wf[0][sm], wf[0][s0], wf[0][sp], wf[1][sm], wf[1][s0], wf[1][sp],
         wf[2][sm], wf[2][s0], wf[2][sp], wf[ft][sm], wf[ft][s0], wf[ft][sp],
         wq[pm], wq[1 - pm], of, TCK_WTW,
         RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
         RB(A.nx_), RB(A.ny_), RB(A.nz_), RB(A.nt),
-        RBC(A.sbx), RBC(A.sby), RBC(A.sbz), RBC(A.sbt), PFS ? (const R *)sbv : (const R *)nullptr,
-        RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat), i, j, k, kb0, smt, ib);
+        SRC ? RBC(A.sbx) : nullptr, SRC ? RBC(A.sby) : nullptr, SRC ? RBC(A.sbz) : nullptr,
+        SRC ? RBC(A.sbt) : nullptr, (const R *)sbv,
+        RBT(A.sax), RBT(A.say), RBT(A.saz), RBT(A.sat), i, j, k, kb0, smt, ib, dtv, (const float *)cvv);
     #undef RB
     #undef RBT
     #undef RBC
@@ -795,8 +825,15 @@ int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
   }
   const dim3 grd((unsigned)A.tm.total), blk(TCK_WAVE, window ? TCK_BLOCK / TCK_WAVE : ty);
   if(window){
-    if(mixed) hipLaunchKernelGGL(tck_predict_w<float>, grd, blk, 0, stream, A);
-    else hipLaunchKernelGGL(tck_predict_w<double>, grd, blk, 0, stream, A);
+    const bool src = A.sbx || A.sby || A.sbz || A.sbt;
+    const bool cm = A.cx || A.cy || A.cz || A.ct;
+    #define TCK_PW(RT) \
+      if(src){ if(cm) hipLaunchKernelGGL((tck_predict_w<RT, true, true>), grd, blk, 0, stream, A); \
+               else hipLaunchKernelGGL((tck_predict_w<RT, true, false>), grd, blk, 0, stream, A); } \
+      else   { if(cm) hipLaunchKernelGGL((tck_predict_w<RT, false, true>), grd, blk, 0, stream, A); \
+               else hipLaunchKernelGGL((tck_predict_w<RT, false, false>), grd, blk, 0, stream, A); }
+    if(mixed){ TCK_PW(float) } else { TCK_PW(double) }
+    #undef TCK_PW
   }else{
     if(mixed) hipLaunchKernelGGL(tck_predict<float>, grd, blk, 0, stream, A);
     else hipLaunchKernelGGL(tck_predict<double>, grd, blk, 0, stream, A);
