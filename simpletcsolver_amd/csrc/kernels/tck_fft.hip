@@ -112,6 +112,22 @@ template <typename T> struct dftr<8, T> {
   }
 };
 
+/* block barrier that orders the LDS accesses only: global loads issued before it stay in
+ * flight and pending global stores are not waited for */
+__device__ __forceinline__ void fft_lds_barrier(){
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+/* the same, and no instruction is scheduled across it: the loads of the pass after it (its
+ * twiddle factors) are not hoisted into the pass before it, which would hold more registers */
+__device__ __forceinline__ void fft_pass_barrier(){
+  __builtin_amdgcn_sched_barrier(0);
+  fft_lds_barrier();
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 /* padded position of element n of a line in LDS: one spare slot after every 8 elements, so
  * that the first Stockham pass, which writes elements R apart from consecutive threads, and
  * the row-wise fills spread over all banks */
@@ -149,7 +165,7 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
       dftr<R, T>::run(v[q], s);
     }
   }
-  __syncthreads();
+  fft_pass_barrier();
   #pragma unroll
   for(int q = 0; q < BPT; q++){
     const int bf = tid + q * nt;
@@ -163,7 +179,7 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
       for(int r = 0; r < R; r++) dst[fpad(d0 + r * Ns)] = v[q][r];
     }
   }
-  __syncthreads();
+  fft_pass_barrier();
 }
 
 /* nb transforms of length P.M stored at buf + b * ld; tw[t] = exp(-2 pi i t / M); each pass
@@ -191,14 +207,6 @@ __device__ __forceinline__ long long fft_tile_of(long long ntiles){
   long long t = blockIdx.x;
   if(ntiles % TCK_NXCD == 0) t = (t % TCK_NXCD) * (ntiles / TCK_NXCD) + t / TCK_NXCD;
   return t;
-}
-
-/* block barrier that orders the LDS accesses only: global loads issued before it stay in
- * flight and pending global stores are not waited for */
-__device__ __forceinline__ void fft_lds_barrier(){
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -292,9 +300,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
   const int ti = A.ti;
   const int M = A.P.M;
   const int nt = blockDim.x;
-  const int nel = ti * g.ny;
   const int ntx = (g.nx + ti - 1) / ti;
-  const long long ntiles = (long long)ntx * g.nz;
+  const int ntiles = ntx * g.nz;
   const double dt = *A.dt;
   const double gamma = c_fft_rka[A.rkstep] + c_fft_rkb[A.rkstep];
   const double norm = 1. * g.ny * g.nz;
@@ -304,61 +311,78 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
   const V * twM = (const V *)A.twM;
   const V * twN = (const V *)A.twN;
   V * S = (V *)A.S;
+  /* element q of this thread in a tile: line t0, row j = r0 + q dr (the block size is a
+   * multiple of the tile width, a power of two no larger than 64) */
+  const int t0i = threadIdx.x % ti, r0i = threadIdx.x / ti, dr = nt / ti;
+  int t0 = t0i, r0 = r0i;
   double pre[PF][6];
-  /* the six velocities of element e of tile tl (zero outside the field) */
-  #define DV_LOAD(tl, e, P) { \
-    const int t_ = (e) % ti, j_ = (e) / ti; \
-    const int i_ = 1 + (int)((tl) % ntx) * ti + t_; \
-    if((e) < nel && i_ <= g.nx){ \
-      const long long o_ = tck_idx(g, i_, j_ + 1, 1 + (int)((tl) / ntx)); \
-      P[0] = A.ux[o_]; P[1] = A.ux[o_ + 1]; P[2] = A.uy[o_]; P[3] = A.uy[o_ + sy]; \
-      P[4] = A.uz[o_]; P[5] = A.uz[o_ + sz]; \
-    }else{ P[0] = P[1] = P[2] = P[3] = P[4] = P[5] = 0.; } }
-  long long tile = blockIdx.x;
+  bool pok[PF];
+  /* the six velocities of element q of tile tl: ok when inside the field; otherwise the loads
+   * read an in-bounds stand-in (cell i = 1, j = 1 of the tile's plane) and the element is put
+   * as 0, so that no load is conditional */
+  #define DV_LOAD(tl, q, P, ok) { \
+    const int j_ = r0 + (q) * dr; \
+    const int i_ = 1 + ((tl) % ntx) * ti + t0; \
+    ok = j_ < g.ny && i_ <= g.nx; \
+    const long long o_ = tck_idx(g, ok ? i_ : 1, ok ? j_ + 1 : 1, 1 + (tl) / ntx); \
+    P[0] = A.ux[o_]; P[1] = A.ux[o_ + 1]; P[2] = A.uy[o_]; P[3] = A.uy[o_ + sy]; \
+    P[4] = A.uz[o_]; P[5] = A.uz[o_ + sz]; }
+  /* elements per thread of a tile (the launch guarantees that they cover it), and how many
+   * are loaded together */
+  constexpr int EPT2 = 2 * EPTK;
+  constexpr int G = 2;
+  int tile = blockIdx.x;
   if(tile < ntiles){
     #pragma unroll
-    for(int q = 0; q < PF; q++){ DV_LOAD(tile, threadIdx.x + q * nt, pre[q]); }
+    for(int q = 0; q < PF; q++){ DV_LOAD(tile, q, pre[q], pok[q]); }
   }
   for(; tile < ntiles; tile += gridDim.x){
-    const int i0 = 1 + (int)(tile % ntx) * ti;
-    const int k = 1 + (int)(tile / ntx);
+    /* the element offsets are recomputed in every tile rather than held across the passes */
+    t0 = t0i; r0 = r0i;
+    asm volatile("" : "+v"(t0), "+v"(r0));
+    const int i0 = 1 + (tile % ntx) * ti;
+    const int k = 1 + tile / ntx;
     /* divergence (assign_input :404-423, same operand order as tck_div_yfft) packed as
      * z_m = x_2m + i x_2m+1 */
-    auto put = [&](const int e, const double * P){
-      const int t = e % ti;
-      const int j0 = e / ti;
-      const int i = i0 + t;
+    const int i = i0 + t0;
+    auto put = [&](const int q, const double * P, const bool ok){
+      const int j0 = r0 + q * dr;
+      if(j0 >= g.ny) return;
       double val = 0.;
-      if(i <= g.nx){
+      if(ok){
         const double div = c.ijdxc[i] * (
             - c.jhxf[i] * P[0] + c.jhxf[i + 1] * P[1]
             - c.jhyc[i] * P[2] + c.jhyc[i] * P[3]
             - c.jhzc[i] * P[4] + c.jhzc[i] * P[5]);
         val = prefactor * div;
       }
-      ((T *)(sm + t * A.ld))[2 * fpad(j0 >> 1) + (j0 & 1)] = T(val);
+      ((T *)(sm + t0 * A.ld))[2 * fpad(j0 >> 1) + (j0 & 1)] = T(val);
     };
+    /* the rest of the tile, G elements (6 G loads) issued before the first is waited for */
     #pragma unroll
-    for(int q = 0; q < PF; q++){
-      const int e = threadIdx.x + q * nt;
-      if(e < nel) put(e, pre[q]);
-    }
-    for(int e = threadIdx.x + PF * nt; e < nel; e += nt){
-      double P[6];
-      DV_LOAD(tile, e, P);
-      put(e, P);
+    for(int q0 = PF; q0 < EPT2; q0 += G){
+      double P[G][6];
+      bool ok[G];
+      #pragma unroll
+      for(int u = 0; u < G; u++) if(q0 + u < EPT2){ DV_LOAD(tile, q0 + u, P[u], ok[u]); }
+      if(q0 == PF){
+        #pragma unroll
+        for(int q = 0; q < PF; q++) put(q, pre[q], pok[q]);
+      }
+      #pragma unroll
+      for(int u = 0; u < G; u++) if(q0 + u < EPT2) put(q0 + u, P[u], ok[u]);
     }
     fft_lds_barrier();
-    if(tile + gridDim.x < ntiles){
+    if(tile + (int)gridDim.x < ntiles){
       #pragma unroll
-      for(int q = 0; q < PF; q++){ DV_LOAD(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
+      for(int q = 0; q < PF; q++){ DV_LOAD(tile + (int)gridDim.x, q, pre[q], pok[q]); }
     }
     lds_fft<T, EPTK>(sm, A.ld, ti, A.P, twM, T(-1.));
     for(int e = threadIdx.x; e < ti * (M + 1); e += nt){
       const int t = e % ti;
       const int kk = e / ti;
-      const int i = i0 + t;
-      if(i > g.nx) continue;
+      const int ii = i0 + t;
+      if(ii > g.nx) continue;
       const V zk = sm[t * A.ld + fpad(kk == M ? 0 : kk)];
       const V zc = cconj(sm[t * A.ld + fpad(kk == 0 ? 0 : M - kk)]);
       const V w = twN[kk];
@@ -366,7 +390,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
       const V od = mk2<T>(T(0.5) * (zk.x - zc.x), T(0.5) * (zk.y - zc.y));
       const V wo = cmul(w, od);
       const V X = mk2<T>(ev.x + wo.y, ev.y - wo.x);
-      tck_st(&S[(long long)(i - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))], X);
+      tck_st(&S[(long long)(ii - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))], X);
     }
     fft_lds_barrier();
   }
@@ -414,8 +438,10 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
 
 /* Persistent form: each block loops over tiles and issues the loads of the first PF of its
  * elements of the next tile before the current tile's transform, so that part of the HBM
- * traffic overlaps the LDS passes (the remaining elements are loaded when the tile starts).
- * Compiled for EPTK elements per thread per pass. */
+ * traffic overlaps the LDS passes; the remaining elements are loaded when the tile starts, all
+ * issued before the first is waited for.  Compiled for EPTK elements per thread per pass: the
+ * launch guarantees that blockDim.x EPTK covers the tile and that blockDim.x is a multiple of
+ * the tile width. */
 #ifndef TCK_ZFFT_PF
 #define TCK_ZFFT_PF 2            /* elements per thread loaded one tile ahead */
 #endif
@@ -425,48 +451,62 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
   V * sm = (V *)fft_smem;
-  const int tz = A.tz, nz = A.nz, nt = blockDim.x, nel = tz * nz;
+  const int tz = A.tz, nz = A.nz, nt = blockDim.x;
   const int ntx = (A.nx + tz - 1) / tz;
-  const long long ntiles = (long long)ntx * A.nky;
+  const int ntiles = ntx * A.nky;
   const long long kstride = (long long)A.nx * A.nky;
+  /* element q of this thread in a tile: line t0, row kz = r0 + q dr (the block size is a
+   * multiple of the tile width, a power of two no larger than 64) */
+  const int t0i = threadIdx.x % tz, r0i = threadIdx.x / tz, dr = nt / tz;
+  int t0 = t0i, r0 = r0i;
   V pre[PF];
-  long long tile = blockIdx.x;
-  #define ZF_ELEM(tl, e, out) { \
-    const int t_ = (e) % tz, kz_ = (e) / tz; \
-    const int ii_ = (int)((tl) % ntx) * tz + t_; \
+  bool pok[PF];
+  int tile = blockIdx.x;
+  /* element q of tile tl: ok when inside the spectrum; otherwise the load reads element 0
+   * (in bounds) and the value is replaced by 0 when put, so that no load is conditional */
+  #define ZF_ELEM(tl, q, out, ok) { \
+    const int kz_ = r0 + (q) * dr; \
+    const int tx_ = (tl) % ntx; \
+    const int ii_ = tx_ * tz + t0; \
+    ok = kz_ < nz && ii_ < A.nx; \
     const V * b_ = (const V *)A.S + (long long)A.nx * ((tl) / ntx); \
-    out = ((e) < nel && ii_ < A.nx) ? tck_ld(&b_[ii_ + kstride * kz_]) : mk2<T>(T(0.), T(0.)); }
+    out = tck_ld(ok ? &b_[ii_ + kstride * kz_] : (const V *)A.S); }
+  #define ZF_PUT(q, v, ok) { const int kz_ = r0 + (q) * dr; \
+    if(kz_ < nz) sm[t0 * A.ld + fpad(kz_)] = (ok) ? (v) : mk2<T>(T(0.), T(0.)); }
   if(tile < ntiles){
     #pragma unroll
-    for(int q = 0; q < PF; q++){ ZF_ELEM(tile, threadIdx.x + q * nt, pre[q]); }
+    for(int q = 0; q < PF; q++){ ZF_ELEM(tile, q, pre[q], pok[q]); }
   }
   for(; tile < ntiles; tile += gridDim.x){
+    /* the element offsets are recomputed in every tile rather than held across the passes */
+    t0 = t0i; r0 = r0i;
+    asm volatile("" : "+v"(t0), "+v"(r0));
+    /* the rest of the tile: all loads issued before the first is waited for */
+    V cur[EPTK - PF];
+    bool cok[EPTK - PF];
     #pragma unroll
-    for(int q = 0; q < PF; q++){
-      const int e = threadIdx.x + q * nt;
-      if(e < nel) sm[(e % tz) * A.ld + fpad(e / tz)] = pre[q];
-    }
-    for(int e = threadIdx.x + PF * nt; e < nel; e += nt){
-      V v;
-      ZF_ELEM(tile, e, v);
-      sm[(e % tz) * A.ld + fpad(e / tz)] = v;
-    }
+    for(int q = PF; q < EPTK; q++){ ZF_ELEM(tile, q, cur[q - PF], cok[q - PF]); }
+    #pragma unroll
+    for(int q = 0; q < PF; q++){ ZF_PUT(q, pre[q], pok[q]); }
+    #pragma unroll
+    for(int q = PF; q < EPTK; q++){ ZF_PUT(q, cur[q - PF], cok[q - PF]); }
     fft_lds_barrier();
-    if(tile + gridDim.x < ntiles){
+    if(tile + (int)gridDim.x < ntiles){
       #pragma unroll
-      for(int q = 0; q < PF; q++){ ZF_ELEM(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
+      for(int q = 0; q < PF; q++){ ZF_ELEM(tile + (int)gridDim.x, q, pre[q], pok[q]); }
     }
     lds_fft<T, EPTK>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
-    const int i0 = (int)(tile % ntx) * tz;
+    const int ii = (tile % ntx) * tz + t0;
     V * base = (V *)A.S + (long long)A.nx * (tile / ntx);
-    for(int e = threadIdx.x; e < nel; e += nt){
-      const int t = e % tz;
-      const int kz = e / tz;
-      if(i0 + t < A.nx) tck_st(&base[i0 + t + kstride * kz], sm[t * A.ld + fpad(kz)]);
+    #pragma unroll
+    for(int q = 0; q < EPTK; q++){
+      const int kz = r0 + q * dr;
+      if(kz < nz && ii < A.nx) tck_st(&base[ii + kstride * kz], sm[t0 * A.ld + fpad(kz)]);
     }
     fft_lds_barrier();
   }
   #undef ZF_ELEM
+  #undef ZF_PUT
 }
 
 /* ------------------------------------------------------------------------------------- */
@@ -574,55 +614,72 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
   const int ti = A.ti;
   const int M = A.P.M;
   const int nt = blockDim.x;
-  const int nel = ti * M;
   const int ntx = (g.nx + ti - 1) / ti;
-  const long long ntiles = (long long)ntx * g.nz;
+  const int ntiles = ntx * g.nz;
   const V * S = (const V *)A.S;
   const V * twN = (const V *)A.twN;
   T * psi = (T *)A.psi;
+  /* element q of this thread in a tile: line t0, wavenumber kk = r0 + q dr (the block size is
+   * a multiple of the tile width, a power of two no larger than 64); only q = 0 can hold
+   * kk = 0 */
+  const int t0i = threadIdx.x % ti, r0i = threadIdx.x / ti, dr = nt / ti;
+  int t0 = t0i, r0 = r0i;
   V pre[PF];
-  long long tile = blockIdx.x;
-  /* raw spectrum value X_kk of element e of tile tl */
-  #define YI_ELEM(tl, e, out) { \
-    const int t_ = (e) % ti, kk_ = (e) / ti; \
-    const int i_ = 1 + (int)((tl) % ntx) * ti + t_; \
+  bool pok[PF];
+  int tile = blockIdx.x;
+  /* raw spectrum value X_kk of element q of tile tl: ok when inside the field; otherwise the
+   * load reads element 0 (in bounds) and the value is replaced by 0 when put, so that no load
+   * is conditional */
+  #define YI_ELEM(tl, q, out, ok) { \
+    const int kk_ = r0 + (q) * dr; \
+    const int i_ = 1 + ((tl) % ntx) * ti + t0; \
+    ok = kk_ < M && i_ <= g.nx; \
     const long long o_ = (long long)(i_ - 1) + (long long)g.nx * ((long long)A.nky * ((tl) / ntx) + kk_); \
-    out = ((e) < nel && i_ <= g.nx) ? tck_ld(&S[o_]) : mk2<T>(T(0.), T(0.)); }
+    out = tck_ld(ok ? &S[o_] : S); }
+  /* X_M of the line of element 0 (used when that element is kk = 0) */
+  #define YI_XM(tl, out) { \
+    const int i_ = 1 + ((tl) % ntx) * ti + t0; \
+    const bool ok_ = r0 == 0 && i_ <= g.nx; \
+    const long long o_ = (long long)(i_ - 1) + (long long)g.nx * ((long long)A.nky * ((tl) / ntx) + M); \
+    out = tck_ld(ok_ ? &S[o_] : S); }
+  V prexm = mk2<T>(T(0.), T(0.));
   if(tile < ntiles){
     #pragma unroll
-    for(int q = 0; q < PF; q++){ YI_ELEM(tile, threadIdx.x + q * nt, pre[q]); }
+    for(int q = 0; q < PF; q++){ YI_ELEM(tile, q, pre[q], pok[q]); }
+    YI_XM(tile, prexm);
   }
   for(; tile < ntiles; tile += gridDim.x){
-    const int i0 = 1 + (int)(tile % ntx) * ti;
-    const int k = 1 + (int)(tile / ntx);
-    const long long rowbase = (long long)g.nx * (long long)A.nky * (k - 1);
+    /* the element offsets are recomputed in every tile rather than held across the passes */
+    t0 = t0i; r0 = r0i;
+    asm volatile("" : "+v"(t0), "+v"(r0));
+    const int i0 = 1 + (tile % ntx) * ti;
+    const int k = 1 + tile / ntx;
     /* X_1..X_{M-1} as loaded; Z'_0 from X_0 and X_M */
-    auto put = [&](const int e, V v){
-      const int t = e % ti;
-      const int kk = e / ti;
-      const int i = i0 + t;
-      if(kk == 0 && i <= g.nx){
-        V xm = tck_ld(&S[(long long)(i - 1) + rowbase + (long long)g.nx * M]);
+    auto put = [&](const int q, V v, const bool ok, V xm){
+      const int kk = r0 + q * dr;
+      if(kk >= M) return;
+      if(!ok) v = mk2<T>(T(0.), T(0.));
+      else if(kk == 0){
         v.y = T(0.);
         xm.y = T(0.);
         v = c2r_pre(v, xm, twN[0]);
       }
-      sm[t * A.ld + fpad(kk)] = v;
+      sm[t0 * A.ld + fpad(kk)] = v;
     };
+    /* the rest of the tile: all loads issued before the first is waited for */
+    V cur[EPTK - PF];
+    bool cok[EPTK - PF];
     #pragma unroll
-    for(int q = 0; q < PF; q++){
-      const int e = threadIdx.x + q * nt;
-      if(e < nel) put(e, pre[q]);
-    }
-    for(int e = threadIdx.x + PF * nt; e < nel; e += nt){
-      V v;
-      YI_ELEM(tile, e, v);
-      put(e, v);
-    }
+    for(int q = PF; q < EPTK; q++){ YI_ELEM(tile, q, cur[q - PF], cok[q - PF]); }
+    #pragma unroll
+    for(int q = 0; q < PF; q++) put(q, pre[q], pok[q], prexm);
+    #pragma unroll
+    for(int q = PF; q < EPTK; q++) put(q, cur[q - PF], cok[q - PF], prexm);
     fft_lds_barrier();
-    if(tile + gridDim.x < ntiles){
+    if(tile + (int)gridDim.x < ntiles){
       #pragma unroll
-      for(int q = 0; q < PF; q++){ YI_ELEM(tile + gridDim.x, threadIdx.x + q * nt, pre[q]); }
+      for(int q = 0; q < PF; q++){ YI_ELEM(tile + (int)gridDim.x, q, pre[q], pok[q]); }
+      YI_XM(tile + (int)gridDim.x, prexm);
     }
     const int half = M / 2;
     for(int e = threadIdx.x; e < ti * half; e += nt){
@@ -651,6 +708,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
     fft_lds_barrier();
   }
   #undef YI_ELEM
+  #undef YI_XM
 }
 
 /* ------------------------------------------------------------------------------------- */
