@@ -369,7 +369,7 @@ int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_so
   if(p->nx < 3 || p->ny < 4 || p->nz < 3) return tc_fail("grid too small: %d x %d x %d", p->nx, p->ny, p->nz);
   if(p->ny % 2 != 0) return tc_fail("ny must be even (real theta transform)");
   if(!tck_fft_supported(p->ny / 2) || !tck_fft_supported(p->nz))
-    return tc_fail("ny/2 = %d and nz = %d must be products of 2 and 3", p->ny / 2, p->nz);
+    return tc_fail("ny/2 = %d and nz = %d must be products of 2, 3, 5 and 7", p->ny / 2, p->nz);
   if(p->nx > 1024) return tc_fail("nx = %d exceeds 1024 (radial line solver)", p->nx);
   if(p->ny / 2 > 4096 || p->nz > 4096) return tc_fail("transform length exceeds 4096");
   if(p->precision < TC_FP64 || p->precision > TC_MIXED_REFINED) return tc_fail("precision %d not supported", p->precision);

@@ -30,6 +30,8 @@
 #define FFT_MAXST 16
 #define FFT_EPT 16   /* complex elements per thread per pass */
 #define FFT_MAXTHREADS 1024
+#define FFT_G 4      /* one-tile transforms: divergence elements whose loads are issued together */
+#define FFT_GZ 8     /* one-tile transforms: spectrum elements whose loads are issued together */
 
 /* complex pair of the transform's arithmetic type T (double, or float in the mixed step) */
 template <typename T> struct cvec;
@@ -91,6 +93,67 @@ template <typename T> struct dftr<3, T> {
     x[2] = csub(m, t2);
   }
 };
+
+/* odd R-point DFT by conjugate pairs (R = 5, 7): with a_n = x_n + x_{R-n} and
+ * b_n = x_n - x_{R-n}, n = 1..h = (R-1)/2,
+ *   y_0 = x_0 + sum a_n,  y_k, y_{R-k} = x_0 + sum_n a_n cos(2 pi n k / R)
+ *                                     +- s i sum_n b_n sin(2 pi n k / R),  k = 1..h */
+template <int R> struct oddtab;
+template <> struct oddtab<5> {
+  static __device__ __forceinline__ double c(int m){
+    return m == 1 ? 0.3090169943749474241023 : -0.8090169943749474241023;
+  }
+  static __device__ __forceinline__ double sn(int m){
+    return m == 1 ? 0.9510565162951535721164 : 0.5877852522924731291687;
+  }
+};
+template <> struct oddtab<7> {
+  static __device__ __forceinline__ double c(int m){
+    return m == 1 ? 0.6234898018587335305250 : m == 2 ? -0.2225209339563144042889 : -0.9009688679024191262361;
+  }
+  static __device__ __forceinline__ double sn(int m){
+    return m == 1 ? 0.7818314824680298087084 : m == 2 ? 0.9749279121818236070181 : 0.4338837391175581204758;
+  }
+};
+
+template <int R, typename T> struct dftodd {
+  typedef typename cvec<T>::type V;
+  static __device__ __forceinline__ void run(V * x, T s){
+    constexpr int h = (R - 1) / 2;
+    V a[h], b[h];
+    #pragma unroll
+    for(int n = 1; n <= h; n++){ a[n - 1] = cadd(x[n], x[R - n]); b[n - 1] = csub(x[n], x[R - n]); }
+    V y0 = x[0];
+    #pragma unroll
+    for(int n = 0; n < h; n++) y0 = cadd(y0, a[n]);
+    V re[h], im[h];
+    #pragma unroll
+    for(int k = 1; k <= h; k++){
+      V r = x[0];
+      V q = mk2<T>(T(0.), T(0.));
+      #pragma unroll
+      for(int n = 1; n <= h; n++){
+        /* m = n k mod R, folded into 1..h with the sign of the sine */
+        const int m0 = (n * k) % R;
+        const int m = m0 <= h ? m0 : R - m0;
+        const T cv = T(oddtab<R>::c(m));
+        const T sv = T(m0 <= h ? oddtab<R>::sn(m) : - oddtab<R>::sn(m));
+        r = mk2<T>(r.x + cv * a[n - 1].x, r.y + cv * a[n - 1].y);
+        q = mk2<T>(q.x + sv * b[n - 1].x, q.y + sv * b[n - 1].y);
+      }
+      re[k - 1] = r;
+      im[k - 1] = crot(q, s);
+    }
+    x[0] = y0;
+    #pragma unroll
+    for(int k = 1; k <= h; k++){
+      x[k] = cadd(re[k - 1], im[k - 1]);
+      x[R - k] = csub(re[k - 1], im[k - 1]);
+    }
+  }
+};
+template <typename T> struct dftr<5, T> : dftodd<5, T> {};
+template <typename T> struct dftr<7, T> : dftodd<7, T> {};
 
 template <typename T> struct dftr<8, T> {
   typedef typename cvec<T>::type V;
@@ -183,8 +246,10 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
 }
 
 /* nb transforms of length P.M stored at buf + b * ld; tw[t] = exp(-2 pi i t / M); each pass
- * keeps at most EPT elements per thread in registers, so blockDim.x * EPT >= nb * P.M */
-template <typename T, int EPT = FFT_EPT>
+ * keeps at most EPT elements per thread in registers, so blockDim.x * EPT >= nb * P.M.  W:
+ * the plan may hold radix 5 and 7 passes (compiled only into the kernels of such lengths, so
+ * that the 2^a 3^b kernels keep their register budget) */
+template <typename T, int EPT = FFT_EPT, bool W = false>
 __device__ void lds_fft(typename cvec<T>::type * buf, int ld, int nb, const FftPlan & P,
     const typename cvec<T>::type * __restrict__ tw, T s){
   int Ns = 1;
@@ -193,7 +258,9 @@ __device__ void lds_fft(typename cvec<T>::type * buf, int ld, int nb, const FftP
     if(R == 8) fft_stage<8, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
     else if(R == 4) fft_stage<4, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
     else if(R == 3) fft_stage<3, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
-    else fft_stage<2, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(W && R == 5) fft_stage<5, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(W && R == 7) fft_stage<7, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
+    else if(R == 2) fft_stage<2, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
     Ns *= R;
   }
 }
@@ -225,7 +292,7 @@ struct YFwdArgs {
 
 /* the divergence is formed in FP64 (assign_input :404-423) and rounded to the transform
  * type T only when it enters the transform */
-template <typename T>
+template <typename T, bool W>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -246,24 +313,39 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
   const long long sz = g.plane;
   const V * twM = (const V *)A.twM;
   const V * twN = (const V *)A.twN;
-  /* divergence packed as z_m = x_2m + i x_2m+1 */
-  for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
-    const int t = e % ti;
-    const int j0 = e / ti;
-    const int i = i0 + t;
-    double val = 0.;
-    if(i <= g.nx){
-      const long long o = tck_idx(g, i, j0 + 1, k);
-      const double div = c.ijdxc[i] * (
-          - c.jhxf[i] * A.ux[o] + c.jhxf[i + 1] * A.ux[o + 1]
-          - c.jhyc[i] * A.uy[o] + c.jhyc[i] * A.uy[o + sy]
-          - c.jhzc[i] * A.uz[o] + c.jhzc[i] * A.uz[o + sz]);
-      val = prefactor * div;
+  /* divergence packed as z_m = x_2m + i x_2m+1; the velocities of FFT_G elements are loaded
+   * together (out-of-field elements read cell (1, 1) of the plane and are put as 0, so that no
+   * load is conditional) */
+  const int nel = ti * g.ny;
+  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_G * blockDim.x){
+    double P[FFT_G][6];
+    #pragma unroll
+    for(int u = 0; u < FFT_G; u++){
+      const int e = e0 + u * blockDim.x;
+      const int t = e % ti, j0 = e / ti, i = i0 + t;
+      const bool ok = e < nel && i <= g.nx;
+      const long long o = tck_idx(g, ok ? i : 1, ok ? j0 + 1 : 1, k);
+      P[u][0] = A.ux[o]; P[u][1] = A.ux[o + 1]; P[u][2] = A.uy[o]; P[u][3] = A.uy[o + sy];
+      P[u][4] = A.uz[o]; P[u][5] = A.uz[o + sz];
     }
-    ((T *)(sm + t * A.ld))[2 * fpad(j0 >> 1) + (j0 & 1)] = T(val);
+    #pragma unroll
+    for(int u = 0; u < FFT_G; u++){
+      const int e = e0 + u * blockDim.x;
+      if(e >= nel) break;
+      const int t = e % ti, j0 = e / ti, i = i0 + t;
+      double val = 0.;
+      if(i <= g.nx){
+        const double div = c.ijdxc[i] * (
+            - c.jhxf[i] * P[u][0] + c.jhxf[i + 1] * P[u][1]
+            - c.jhyc[i] * P[u][2] + c.jhyc[i] * P[u][3]
+            - c.jhzc[i] * P[u][4] + c.jhzc[i] * P[u][5]);
+        val = prefactor * div;
+      }
+      ((T *)(sm + t * A.ld))[2 * fpad(j0 >> 1) + (j0 & 1)] = T(val);
+    }
   }
   __syncthreads();
-  lds_fft<T>(sm, A.ld, ti, A.P, twM, T(-1.));
+  lds_fft<T, FFT_EPT, W>(sm, A.ld, ti, A.P, twM, T(-1.));
   /* split: X_k = (Z_k + conj Z_{M-k}) / 2 - i W^k (Z_k - conj Z_{M-k}) / 2, k = 0..M */
   V * S = (V *)A.S;
   for(int e = threadIdx.x; e < ti * (M + 1); e += blockDim.x){
@@ -290,7 +372,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
 #define TCK_DIVY_PF 1
 #endif
 
-template <typename T, int PF, int EPTK>
+template <typename T, int PF, int EPTK, bool W>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -377,7 +459,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
       #pragma unroll
       for(int q = 0; q < PF; q++){ DV_LOAD(tile + (int)gridDim.x, q, pre[q], pok[q]); }
     }
-    lds_fft<T, EPTK>(sm, A.ld, ti, A.P, twM, T(-1.));
+    lds_fft<T, EPTK, W>(sm, A.ld, ti, A.P, twM, T(-1.));
     for(int e = threadIdx.x; e < ti * (M + 1); e += nt){
       const int t = e % ti;
       const int kk = e / ti;
@@ -407,7 +489,7 @@ struct ZArgs {
 };
 
 /* one block per tile of tz consecutive i at one ky */
-template <typename T>
+template <typename T, bool W>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -420,14 +502,26 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   const int ky = (int)(tile / ntx);
   const long long kstride = (long long)A.nx * A.nky;
   V * base = (V *)A.S + (long long)A.nx * ky;
-  for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
-    const int t = e % tz;
-    const int kz = e / tz;
-    const int ii = i0 + t;
-    sm[t * A.ld + fpad(kz)] = ii < A.nx ? tck_ld(&base[ii + kstride * kz]) : mk2<T>(T(0.), T(0.));
+  /* FFT_GZ elements loaded together; out-of-range ones read element 0 and are put as 0 */
+  const int nel = tz * nz;
+  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_GZ * blockDim.x){
+    V v[FFT_GZ];
+    #pragma unroll
+    for(int u = 0; u < FFT_GZ; u++){
+      const int e = e0 + u * blockDim.x;
+      const int t = e % tz, kz = e / tz, ii = i0 + t;
+      v[u] = tck_ld((e < nel && ii < A.nx) ? &base[ii + kstride * kz] : (const V *)A.S);
+    }
+    #pragma unroll
+    for(int u = 0; u < FFT_GZ; u++){
+      const int e = e0 + u * blockDim.x;
+      if(e >= nel) break;
+      const int t = e % tz, kz = e / tz;
+      sm[t * A.ld + fpad(kz)] = i0 + t < A.nx ? v[u] : mk2<T>(T(0.), T(0.));
+    }
   }
   __syncthreads();
-  lds_fft<T>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
+  lds_fft<T, FFT_EPT, W>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
   for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
     const int t = e % tz;
     const int kz = e / tz;
@@ -446,7 +540,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
 #define TCK_ZFFT_PF 2            /* elements per thread loaded one tile ahead */
 #endif
 
-template <typename T, int PF, int EPTK>
+template <typename T, int PF, int EPTK, bool W>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -495,7 +589,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
       #pragma unroll
       for(int q = 0; q < PF; q++){ ZF_ELEM(tile + (int)gridDim.x, q, pre[q], pok[q]); }
     }
-    lds_fft<T, EPTK>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
+    lds_fft<T, EPTK, W>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
     const int ii = (tile % ntx) * tz + t0;
     V * base = (V *)A.S + (long long)A.nx * (tile / ntx);
     #pragma unroll
@@ -534,7 +628,7 @@ __device__ __forceinline__ V c2r_pre(V xk, V xmk, V w){
   return r;
 }
 
-template <typename T>
+template <typename T, bool W>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -549,25 +643,38 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
   const long long rowbase = (long long)g.nx * (long long)A.nky * (k - 1);
   const V * S = (const V *)A.S;
   const V * twN = (const V *)A.twN;
-  /* load X_1..X_{M-1}; Z'_0 needs X_0 and X_M only and is formed here */
-  for(int e = threadIdx.x; e < ti * M; e += blockDim.x){
-    const int t = e % ti;
-    const int kk = e / ti;
-    const int i = i0 + t;
-    V v = mk2<T>(T(0.), T(0.));
-    if(i <= g.nx){
+  /* load X_1..X_{M-1}; Z'_0 needs X_0 and X_M only and is formed here.  FFT_GZ elements are
+   * loaded together (with X_M for a kk = 0 element); out-of-field elements read element 0 and
+   * are put as 0 */
+  const int nel = ti * M;
+  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_GZ * blockDim.x){
+    V v[FFT_GZ], xm[FFT_GZ];
+    #pragma unroll
+    for(int u = 0; u < FFT_GZ; u++){
+      const int e = e0 + u * blockDim.x;
+      const int t = e % ti, kk = e / ti, i = i0 + t;
+      const bool ok = e < nel && i <= g.nx;
       const long long o = (long long)(i - 1) + rowbase;
-      if(kk == 0){
-        V x0 = tck_ld(&S[o]);
-        V xm = tck_ld(&S[o + (long long)g.nx * M]);
-        x0.y = T(0.);
-        xm.y = T(0.);
-        v = c2r_pre(x0, xm, twN[0]);
-      }else{
-        v = tck_ld(&S[o + (long long)g.nx * kk]);
-      }
+      v[u] = tck_ld(ok ? &S[o + (long long)g.nx * kk] : S);
+      xm[u] = tck_ld((ok && kk == 0) ? &S[o + (long long)g.nx * M] : S);
     }
-    sm[t * A.ld + fpad(kk)] = v;
+    #pragma unroll
+    for(int u = 0; u < FFT_GZ; u++){
+      const int e = e0 + u * blockDim.x;
+      if(e >= nel) break;
+      const int t = e % ti, kk = e / ti, i = i0 + t;
+      V w = mk2<T>(T(0.), T(0.));
+      if(i <= g.nx){
+        w = v[u];
+        if(kk == 0){
+          V x0 = v[u], xmm = xm[u];
+          x0.y = T(0.);
+          xmm.y = T(0.);
+          w = c2r_pre(x0, xmm, twN[0]);
+        }
+      }
+      sm[t * A.ld + fpad(kk)] = w;
+    }
   }
   __syncthreads();
   /* pairs (k, M-k), k = 1..M/2, in place */
@@ -586,7 +693,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     sm[t * A.ld + fpad(k1)] = z1;
   }
   __syncthreads();
-  lds_fft<T>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
+  lds_fft<T, FFT_EPT, W>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
   T * psi = (T *)A.psi;
   for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
     const int t = e % ti;
@@ -605,7 +712,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
 #define TCK_YIFFT_PF 2
 #endif
 
-template <typename T, int PF, int EPTK>
+template <typename T, int PF, int EPTK, bool W>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -696,7 +803,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
       sm[t * A.ld + fpad(k1)] = z1;
     }
     fft_lds_barrier();
-    lds_fft<T, EPTK>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
+    lds_fft<T, EPTK, W>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
     for(int e = threadIdx.x; e < ti * g.ny; e += nt){
       const int t = e % ti;
       const int jj = e / ti;
@@ -724,6 +831,8 @@ static int make_plan(int M, FftPlan * P){
     else if(n % 4 == 0) r = 4;
     else if(n % 2 == 0) r = 2;
     else if(n % 3 == 0) r = 3;
+    else if(n % 5 == 0) r = 5;
+    else if(n % 7 == 0) r = 7;
     else return -1;
     if(P->nst >= FFT_MAXST) return -1;
     P->rad[P->nst++] = r;
@@ -735,6 +844,12 @@ static int make_plan(int M, FftPlan * P){
 int tck_fft_supported(int n){
   FftPlan P;
   return make_plan(n, &P) == 0;
+}
+
+/* the plan holds a radix 5 or 7 pass: the launch takes the kernels compiled with them */
+static int plan_wide(const FftPlan * P){
+  for(int q = 0; q < P->nst; q++) if(P->rad[q] == 5 || P->rad[q] == 7) return 1;
+  return 0;
 }
 
 /* Tile of TB transforms of length M per block: TB = 16 (one 128 B line of doubles, or two
@@ -848,6 +963,14 @@ static int g_pc_n = 0;
 
 }   /* extern "C" */
 
+/* sets the kernel's LDS limit, then launches (lambda) */
+template <typename F>
+static int fft_one(const void * fn, F launch, size_t lds){
+  if(set_lds(fn, lds)) return 1;
+  launch();
+  return 0;
+}
+
 template <typename Args>
 static int launch_persistent(void (*kern)(const Args), const Args & A, int nt, size_t lds,
     long long elems, long long ntiles, hipStream_t stream){
@@ -872,6 +995,14 @@ static int launch_persistent(void (*kern)(const Args), const Args & A, int nt, s
 
 extern "C" {
 
+/* the one-tile form of kernel K for the transform type (float when mixed) and radix set;
+ * nonzero when the LDS limit could not be raised */
+#define FFT_ONE(K, mixed, wide) ( \
+  (mixed) ? ((wide) ? fft_one((const void *)K<float, true>, [&]{ hipLaunchKernelGGL((K<float, true>), grd, dim3(nt), lds, stream, A); }, lds) \
+                    : fft_one((const void *)K<float, false>, [&]{ hipLaunchKernelGGL((K<float, false>), grd, dim3(nt), lds, stream, A); }, lds)) \
+          : ((wide) ? fft_one((const void *)K<double, true>, [&]{ hipLaunchKernelGGL((K<double, true>), grd, dim3(nt), lds, stream, A); }, lds) \
+                    : fft_one((const void *)K<double, false>, [&]{ hipLaunchKernelGGL((K<double, false>), grd, dim3(nt), lds, stream, A); }, lds)))
+
 /* mixed: spectrum, transforms and psi in FP32 (twiddles twM, twN, tw as float2) */
 int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
     const double * uy, const double * uz, const double * dt, int rkstep, void * S,
@@ -890,19 +1021,14 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
 #if TCK_DIVY_PF > 0
   {
     const long long el = (long long)A.ti * A.P.M;
-    const int r = mixed ? 0 : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT>, A, nt, lds, el, ntiles, stream);
+    const int r = mixed ? 0 : (plan_wide(&A.P) ? launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, true>, A, nt, lds, el, ntiles, stream)
+                                   : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
 #endif
   const dim3 grd((unsigned)ntiles);
-  if(mixed){
-    if(set_lds((const void *)tck_div_yfft<float>, lds)) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(tck_div_yfft<float>, grd, dim3(nt), lds, stream, A);
-  }else{
-    if(set_lds((const void *)tck_div_yfft<double>, lds)) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(tck_div_yfft<double>, grd, dim3(nt), lds, stream, A);
-  }
+  if(FFT_ONE(tck_div_yfft, mixed, plan_wide(&A.P))) return (int)hipErrorInvalidValue;
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -921,19 +1047,14 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
 #if TCK_ZFFT_PF > 0
   {
     const long long el = (long long)A.tz * nz;
-    const int r = mixed ? 0 : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT>, A, nt, lds, el, ntiles, stream);
+    const int r = mixed ? 0 : (plan_wide(&A.P) ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, true>, A, nt, lds, el, ntiles, stream)
+                                   : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
 #endif
   const dim3 grd((unsigned)ntiles);
-  if(mixed){
-    if(set_lds((const void *)tck_zfft<float>, lds)) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(tck_zfft<float>, grd, dim3(nt), lds, stream, A);
-  }else{
-    if(set_lds((const void *)tck_zfft<double>, lds)) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(tck_zfft<double>, grd, dim3(nt), lds, stream, A);
-  }
+  if(FFT_ONE(tck_zfft, mixed, plan_wide(&A.P))) return (int)hipErrorInvalidValue;
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -953,19 +1074,14 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
 #if TCK_YIFFT_PF > 0
   {
     const long long el = (long long)A.ti * A.P.M;
-    const int r = mixed ? 0 : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT>, A, nt, lds, el, ntiles, stream);
+    const int r = mixed ? 0 : (plan_wide(&A.P) ? launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, true>, A, nt, lds, el, ntiles, stream)
+                                   : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
 #endif
   const dim3 grd((unsigned)ntiles);
-  if(mixed){
-    if(set_lds((const void *)tck_yifft_psi<float>, lds)) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(tck_yifft_psi<float>, grd, dim3(nt), lds, stream, A);
-  }else{
-    if(set_lds((const void *)tck_yifft_psi<double>, lds)) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(tck_yifft_psi<double>, grd, dim3(nt), lds, stream, A);
-  }
+  if(FFT_ONE(tck_yifft_psi, mixed, plan_wide(&A.P))) return (int)hipErrorInvalidValue;
   TCK_CHECK_LAUNCH();
   return 0;
 }

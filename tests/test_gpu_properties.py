@@ -2,7 +2,7 @@
 
 * a manufactured potential: u = gamma dt grad(psi_m) (the gradient of correct/*.c) must come
   back from tc_potential as psi_m up to a constant, at sizes where the FFT lengths take every
-  radix of the device FFT (2, 3, 4, 8) and an odd radial length;
+  radix of the device FFT (2, 3, 4, 5, 7, 8) and an odd radial length;
 * a step leaves a solenoidal field (max |div u| at round-off, logging/divergence.c);
 * plane Couette stays an exact steady state;
 * running statistics (statistics.c) equal the oracle's after steps and samples in between;
@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("curved", [True, False])
-@pytest.mark.parametrize("shape", [(16, 12, 8), (9, 24, 6), (33, 16, 48)])
+@pytest.mark.parametrize("shape", [(16, 12, 8), (9, 24, 6), (33, 16, 48), (12, 20, 14),
+                                   (10, 70, 35), (7, 112, 50)])
 def test_device_manufactured_potential(need_gpu, curved, shape):
     nx, ny, nz = shape
     case = make_case(nx=nx, ny=ny, nz=nz, Re=100.0, is_curved=curved, amp=0.0, uy_walls=(0.0, 0.0))
@@ -43,6 +44,60 @@ def test_device_manufactured_potential(need_gpu, curved, shape):
     err = np.abs(d - d.mean()).max() / np.abs(psi).max()
     print(f"manufactured psi {shape} curved={curved}: {err:.3e}")
     assert err <= 1e-11
+    s.close()
+
+
+def test_device_manufactured_potential_radix_5_7_long_lines(need_gpu):
+    """lengths with radix 5 and 7 passes at a size where the transforms take their persistent
+    kernels: 128 x 200 x 700 (theta: 100 = 4 * 25, z: 700 = 4 * 25 * 7)"""
+    _manufactured_long(128, 200, 700, 31)
+
+
+def _manufactured_long(nx, ny, nz, seed):
+    from simpletcsolver_amd import Solver
+    small = make_case(nx=nx, ny=ny, nz=4, Re=100.0, amp=0.0, uy_walls=(0.0, 0.0))
+    o = oracle_for(small)
+    hxxf, hyxc = o.array("hxxf").copy(), o.array("hyxc").copy()
+    o.close()
+    hz = small["kw"]["lengths"][2] / nz
+    rng = np.random.default_rng(seed)
+    psi = rng.standard_normal((nz, ny, nx))
+    rk, dt = 1, 0.13
+    g = orc.rkcoefs()[rk, 2] * dt
+    ux = np.zeros((nz, ny, nx + 1))
+    ux[:, :, 1:nx] = g * (psi[:, :, 1:] - psi[:, :, :-1]) / hxxf[1:nx]
+    uy = np.zeros((nz, ny, nx + 2))
+    uy[:, :, 1:nx + 1] = g * (psi - np.roll(psi, 1, axis=1)) / hyxc[1:nx + 1]
+    uz = np.zeros((nz, ny, nx + 2))
+    uz[:, :, 1:nx + 1] = g * (psi - np.roll(psi, 1, axis=0)) / hz
+    s = Solver(nx, ny, nz, small["xf"], small["xc"], **small["kw"])
+    s.load_fields({"ux": ux, "uy": uy, "uz": uz})
+    s.potential(rk, dt)
+    got = s.get_field("psi")[:, :, 1:-1]
+    s.close()
+    d = got - psi
+    err = np.abs(d - d.mean()).max() / np.abs(psi).max()
+    print(f"manufactured psi {(nx, ny, nz)}: {err:.3e}")
+    assert err <= 1e-9
+
+
+@pytest.mark.parametrize("shape,implicit,scalar", [((12, 20, 14), (1, 1, 1), 1),
+                                                   ((10, 70, 35), (0, 0, 0), 0)])
+def test_device_steps_with_radix_5_and_7_match_oracle(need_gpu, shape, implicit, scalar):
+    """ten steps on grids whose transforms take radix 5 and 7 passes, against the oracle's
+    mixed-radix transforms, to the ten-step bar of the parity tests"""
+    from test_gpu_parity import VEL, _cmp
+    nx, ny, nz = shape
+    case = make_case(nx=nx, ny=ny, nz=nz, Re=200.0, implicit=implicit, scalar=scalar,
+                     t_amp=1e-2 if scalar else 0.0)
+    o, s = oracle_for(case), gpu_for(case)
+    for n in range(10):
+        dto = o.integrate()
+        dts = s.step()
+        assert abs(dts - dto) <= 1e-15 * dto, (n, dts, dto)
+    _cmp(o, s, VEL + (("t",) if scalar else ()) + ("p", "psi"), case, 1e-10, gauge=("p", "psi"))
+    assert s.divergence() <= 1e-12
+    o.close()
     s.close()
 
 
