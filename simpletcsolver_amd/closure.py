@@ -38,6 +38,38 @@ def uniform_grid(nx, lx, is_curved):
     return xf, xc
 
 
+def clustered_grid(nx, lx, is_curved, stretch, r):
+    """a tanh-clustered radial grid (tc_ic_grid's) of nx cells whose every r-th face is kept
+    by the coarse grid: the fine faces xf[0..nx], centres xc[0..nx+1] and the coarse faces and
+    centres made of the unions of r consecutive fine cells"""
+    from . import ic_grid
+    xf, xc = ic_grid(nx, lx, is_curved, stretch)
+    xfc = xf[::r].copy()
+    nc = nx // r
+    xcc = np.empty(nc + 2)
+    xcc[0] = xfc[0]
+    xcc[1:nc + 1] = 0.5 * xfc[:-1] + 0.5 * xfc[1:]
+    xcc[nc + 1] = xfc[nc]
+    return xf, xc, xfc, xcc
+
+
+def radial_weights(xf, is_curved):
+    """volume weight of each fine cell in x (its width, times its centre radius when curved):
+    the box filter's weights along x on a non-uniform radial grid"""
+    h = xf[1:] - xf[:-1]
+    return h * (0.5 * xf[1:] + 0.5 * xf[:-1]) if is_curved else h
+
+
+def _xblocks(a, r, w):
+    """weighted mean over consecutive blocks of r along the last axis (weights w per cell)"""
+    n = a.shape[-1]
+    if n % r:
+        raise ValueError(f"axis of length {n} is not a multiple of {r}")
+    ww = w.reshape(n // r, r)
+    b = a.reshape(a.shape[:-1] + (n // r, r))
+    return (b * ww).sum(axis=-1) / ww.sum(axis=-1)
+
+
 def _blocks(a, r, axes):
     """mean over consecutive blocks of r along each of axes"""
     shp = []
@@ -53,8 +85,10 @@ def _blocks(a, r, axes):
     return b.mean(axis=red)
 
 
-def box_filter(f, r):
-    """fine fields (arrays [z][y][x], x halos included, as Solver.fields) -> coarse fields"""
+def box_filter(f, r, wx=None):
+    """fine fields (arrays [z][y][x], x halos included, as Solver.fields) -> coarse fields.
+    wx: volume weights of the fine cells along x (radial_weights) for a non-uniform radial
+    grid; None: equal weights (uniform grid)"""
     out = {}
     for name, a in f.items():
         if name == "ux":
@@ -62,14 +96,19 @@ def box_filter(f, r):
             continue
         inner = a[:, :, 1:-1]
         walls = a[:, :, [0, -1]]
+        if wx is not None:
+            inner = _xblocks(inner, r, wx)
+            xr = 1
+        else:
+            xr = r
         if name == "uy":
-            ci = _blocks(inner[:, ::r, :], r, (0, 2))
+            ci = _blocks(inner[:, ::r, :], r, (0,)) if xr == 1 else _blocks(inner[:, ::r, :], r, (0, 2))
             cw = _blocks(walls[:, ::r, :], r, (0,))
         elif name == "uz":
-            ci = _blocks(inner[::r, :, :], r, (1, 2))
+            ci = _blocks(inner[::r, :, :], r, (1,)) if xr == 1 else _blocks(inner[::r, :, :], r, (1, 2))
             cw = _blocks(walls[::r, :, :], r, (1,))
         else:
-            ci = _blocks(inner, r, (0, 1, 2))
+            ci = _blocks(inner, r, (0, 1)) if xr == 1 else _blocks(inner, r, (0, 1, 2))
             cw = _blocks(walls, r, (0, 1))
         out[name] = np.concatenate([cw[:, :, :1], ci, cw[:, :, 1:]], axis=2)
     return out
@@ -95,16 +134,24 @@ class Pair:
     """a fine solver and the coarse solver of the same flow (box filter r in every direction)"""
 
     def __init__(self, nx, ny, nz, r, lengths, Re, Pr=1.0, is_curved=True, scalar=1,
-                 implicit=(1, 0, 0), device=0):
+                 implicit=(1, 0, 0), device=0, stretch=0.0):
         self.r = r
         self.nxc, self.nyc, self.nzc = nx // r, ny // r, nz // r
         if (self.nxc * r, self.nyc * r, self.nzc * r) != (nx, ny, nz):
             raise ValueError("the fine grid must be a multiple of the filter width")
         kw = dict(is_curved=is_curved, lengths=lengths, Re=Re, Pr=Pr, implicit=implicit,
                   scalar=scalar, device=device)
-        xf, xc = uniform_grid(nx, lengths[0], is_curved)
+        if stretch > 0.0:
+            # radial cells clustered toward both walls; a coarse cell is the union of r fine
+            # cells, and the filter weights them by volume
+            xf, xc, self.xfc, self.xcc = clustered_grid(nx, lengths[0], is_curved, stretch, r)
+            self.wx = radial_weights(xf, is_curved)
+        else:
+            xf, xc = uniform_grid(nx, lengths[0], is_curved)
+            self.xfc, self.xcc = uniform_grid(self.nxc, lengths[0], is_curved)
+            self.wx = None
+        self.xf, self.xc = xf, xc
         self.fine = Solver(nx, ny, nz, xf, xc, **kw)
-        self.xfc, self.xcc = uniform_grid(self.nxc, lengths[0], is_curved)
         self.coarse = Solver(self.nxc, self.nyc, self.nzc, self.xfc, self.xcc, **kw)
         self.scalar = scalar
 
@@ -151,7 +198,7 @@ def generate(pair, n_samples, spacing, dt_coarse, states=None, log=None, torques
     for m in range(n_samples):
         if log and m % 8 == 0:
             log(f"sample {m} of {n_samples}")
-        f0 = box_filter(pair.fine_state(), r)
+        f0 = box_filter(pair.fine_state(), r, pair.wx)
         for q in range(r):
             if q == 0 and torques is not None:
                 pair.fine.arm_diagnostics(True)
@@ -159,7 +206,7 @@ def generate(pair, n_samples, spacing, dt_coarse, states=None, log=None, torques
             if q == 0 and torques is not None:
                 tf = float(pair.fine.injection()[0])
                 pair.fine.arm_diagnostics(False)
-        f1 = box_filter(pair.fine_state(), r)
+        f1 = box_filter(pair.fine_state(), r, pair.wx)
         tc = [] if torques is not None else None
         s1 = pair.coarse_step(f0, dt_coarse, torque=tc)
         if torques is not None:

@@ -20,3 +20,28 @@ def test_box_filter_is_consistent_with_the_staggering():
     assert np.isclose(c["t"][1, 2, 3], f["t"][4:8, 8:12, 9:13].mean())
     assert np.isclose(c["t"][1, 2, 0], f["t"][4:8, 8:12, 0].mean())
     assert np.isclose(c["t"][1, 2, -1], f["t"][4:8, 8:12, -1].mean())
+
+
+def test_box_filter_weights_cells_by_volume_on_a_clustered_grid():
+    """on a non-uniform radial grid the coarse centre value is the volume-weighted mean of its
+    r fine cells (width times radius: the exact integral of r dr over the cell), so the fine
+    cells' exact means of a field filter to the exact mean over the coarse cell; face
+    variables are unweighted"""
+    from simpletcsolver_amd import closure
+    nz, ny, nx, r = 4, 8, 16, 4
+    xf = 1.0 + 0.4 * (1.0 - np.cos(np.pi * np.arange(nx + 1) / nx)) / 2.0
+    w = closure.radial_weights(xf, True)
+    assert np.allclose(w, (xf[1:] - xf[:-1]) * 0.5 * (xf[1:] + xf[:-1]))
+    def cell_mean(a, b):                     # exact mean of r^2 / 2 over [a, b], weight r dr
+        return ((b ** 4 - a ** 4) / 8.0) / ((b ** 2 - a ** 2) / 2.0)
+    fine = cell_mean(xf[:-1], xf[1:])
+    t = np.zeros((nz, ny, nx + 2))
+    t[:, :, 1:-1] = fine
+    c = closure.box_filter({"t": t}, r, w)
+    xfc = xf[::r]
+    assert np.allclose(c["t"][0, 0, 1:-1], cell_mean(xfc[:-1], xfc[1:]), rtol=1e-13)
+    # face variables are sampled on the coarse faces and averaged over y and z only
+    rng = np.random.default_rng(1)
+    ux = rng.standard_normal((nz, ny, nx + 1))
+    cu = closure.box_filter({"ux": ux}, r, w)
+    assert np.isclose(cu["ux"][0, 1, 2], ux[0:4, 4:8, 8].mean())
