@@ -268,6 +268,33 @@ def evaluate_loss(net, norm, x, y):
     return tot / max(cnt, 1)
 
 
+def model_power(solver, net1, norm, xf, xc, lengths, is_curved=True):
+    """the kinetic-energy input rate of the coupled closure at the solver's current fields:
+    sum over the cells of J u . c, with c the corrections the solver adds to du/dt (ux at the
+    faces i = 2..nx as the mean of the two neighbouring centres, as the device scatter forms
+    them), in the units of the solver's injection and dissipation (energy per unit time, the
+    J-weighted sums of src/logging/*.c).  Negative: the closure removes energy, as a subgrid
+    dissipation does.  net1: the batch-1 network coupled to the solver; xf, xc: the solver's
+    radial faces and centres"""
+    f = solver.fields()
+    xf, xc = np.asarray(xf), np.asarray(xc)
+    nz, ny, nx = f["uy"].shape[0], f["uy"].shape[1], f["uy"].shape[2] - 2
+    y = net1.predict(norm.x(network_inputs(f, xc)[None]))[0] * norm.out_scale
+    ly, lz = lengths[1], lengths[2]
+    hz = lz / nz
+    hxxf = xc[1:] - xc[:-1]                        # HXXF(i), i = 1..nx+1
+    hxxc = xf[1:] - xf[:-1]                        # HXXC(i), i = 1..nx
+    hyxf = xf * ly / ny if is_curved else np.full(nx + 1, ly / ny)
+    hyxc = xc[1:-1] * ly / ny if is_curved else np.full(nx, ly / ny)
+    jf = hxxf * hyxf * hz                          # JDXF(i), i = 1..nx+1
+    jc = hxxc * hyxc * hz                          # JDXC(i), i = 1..nx
+    cx = 0.5 * y[:, :, :-1, 0] + 0.5 * y[:, :, 1:, 0]    # faces i = 2..nx
+    p = (f["ux"][:, :, 1:nx] * cx * jf[1:nx]).sum()
+    p += (f["uy"][:, :, 1:-1] * y[..., 1] * jc).sum()
+    p += (f["uz"][:, :, 1:-1] * y[..., 2] * jc).sum()
+    return float(p)
+
+
 def attach(solver, net, norm, device=0):
     """couple a trained network (any batch size) to a solver of the network's grid; returns the
     batch-1 network now coupled (keep it alive while the solver steps)"""

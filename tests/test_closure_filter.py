@@ -45,3 +45,38 @@ def test_box_filter_weights_cells_by_volume_on_a_clustered_grid():
     ux = rng.standard_normal((nz, ny, nx + 1))
     cu = closure.box_filter({"ux": ux}, r, w)
     assert np.isclose(cu["ux"][0, 1, 2], ux[0:4, 4:8, 8].mean())
+
+
+def test_model_power_is_the_j_weighted_work_of_the_corrections():
+    """closure.model_power with a stand-in network of constant outputs: the ux correction at a
+    face is the mean of its two neighbouring centres (faces 2..nx), so with unit outputs and
+    ux = 1 on the faces the power is the sum of JDXF over the inner faces, plus the centre
+    terms with uy = 2, uz = 3"""
+    from simpletcsolver_amd import closure
+    nz, ny, nx = 4, 6, 8
+    lengths = (0.4, 1.0, 2.0)
+    xf = 1.0 + 0.4 * np.linspace(0.0, 1.0, nx + 1) ** 1.3
+    xc = np.concatenate([[xf[0]], 0.5 * (xf[:-1] + xf[1:]), [xf[-1]]])
+
+    class Fields:
+        def fields(self):
+            return {"ux": np.ones((nz, ny, nx + 1)), "uy": 2.0 * np.ones((nz, ny, nx + 2)),
+                    "uz": 3.0 * np.ones((nz, ny, nx + 2)), "t": np.zeros((nz, ny, nx + 2))}
+
+    class Net:
+        def predict(self, x):
+            assert x.shape == (1, nz, ny, nx, 5)
+            return np.ones((1, nz, ny, nx, 4), dtype=np.float32)
+
+    class Norm:
+        out_scale = np.array([0.5, 0.25, 0.125, 1.0], dtype=np.float32)
+
+        def x(self, x):
+            return x
+
+    p = closure.model_power(Fields(), Net(), Norm(), xf, xc, lengths)
+    hz = lengths[2] / nz
+    jf = (xc[1:] - xc[:-1]) * (xf * lengths[1] / ny) * hz
+    jc = (xf[1:] - xf[:-1]) * (xc[1:-1] * lengths[1] / ny) * hz
+    want = ny * nz * (0.5 * jf[1:nx].sum() + (2.0 * 0.25 + 3.0 * 0.125) * jc.sum())
+    assert np.isclose(p, want, rtol=1e-12)
