@@ -338,8 +338,8 @@ static int setup_linesolve(tc_solver_t * s){
   return 0;
 }
 
-/* launch shapes of the cell kernels (planes per block kb, rows per block ty): 32 x 4 for the
- * predictor and 8 x 4 for the correction, shortened until the grid has at least four blocks
+/* launch shapes of the cell kernels (planes per block kb, rows per block ty): 32 x 4 for both
+ * the predictor and the correction, shortened until the grid has at least four blocks
  * per CU, so that small grids fill the device (tc_autotune refines them).  The predictor
  * stages the fields' planes in LDS whenever its tiles have 4 rows. */
 static void default_shapes(tc_solver_t * s){
@@ -347,7 +347,7 @@ static void default_shapes(tc_solver_t * s){
   if(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->p.device) != hipSuccess || ncu <= 0) ncu = 256;
   const long long want = 4LL * ncu;
   const long long nbx = (s->p.nx + 63) / 64;
-  const int start[2][2] = {{32, 4}, {8, 4}};
+  const int start[2][2] = {{32, 4}, {32, 4}};
   const int which[2] = {TC_TUNE_PREDICT, TC_TUNE_CORRECT};
   for(int q = 0; q < 2; q++){
     int kb = start[q][0], ty = start[q][1];

@@ -101,6 +101,31 @@ __device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i
   }
 }
 
+/* Raw buffer access: a resource (base and size in bytes, in scalar registers, uniform across
+ * the wave) plus a 32-bit per-lane byte offset and a uniform one, so that an access costs one
+ * vector register of address.  Word 3 of the resource is the untyped-access value of gfx9
+ * parts.  Accesses at or past the size read 0 and write nothing. */
+#define TCK_BUF_WORD3 0x00020000
+typedef unsigned int tck_u2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tck_rsrc(const void * base, unsigned bytes){
+  return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, TCK_BUF_WORD3);
+}
+template <typename T> __device__ __forceinline__ T tck_bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ double tck_bld<double>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+template <> __device__ __forceinline__ float tck_bld<float>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+/* store of a streamed result (cache policy as tck_st) */
+__device__ __forceinline__ void tck_bst(__amdgpu_buffer_rsrc_t r, unsigned vo, double v){
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, 0, TCK_NT_STORE ? 2 : 0);
+}
+/* store of a halo image (plain, as tck_store_halo's) */
+__device__ __forceinline__ void tck_bsti(__amdgpu_buffer_rsrc_t r, unsigned vo, double v){
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, 0, 0);
+}
+
 /* Store for the second neighbour case ny == 1 or nz == 1 is not supported (ny, nz >= 3 is
  * checked on the host). */
 

@@ -624,8 +624,16 @@ struct CorrectCol {
   double fx, fy, fz;        /* gamma dt / HXXF(i), gamma dt / HYXC(i), gamma dt / hz */
   double hx, hy;            /* HXXF(i), HYXC(i) (CFL bounds) */
   double qf;                /* 2 f / JDXC(i), f = gamma dt nu / 2 */
-  double xl, xc, xu, yl, yc, zl, zc;   /* implicit-direction Laplacian of psi */
+  double xl, xu, yl, zl;   /* implicit-direction Laplacian of psi (centre: - l - u) */
 };
+
+/* gamma dt / hz (uniform) */
+__device__ __forceinline__ double correct_col_fz(const CorrectArgs & A){
+  const int s = A.rkstep;
+  const double gamma = c_rka[s] + c_rkb[s];
+  const double prefactor = gamma * *A.dt;
+  return prefactor / A.c.hz;
+}
 
 __device__ __forceinline__ CorrectCol correct_col(const CorrectArgs & A, int i){
   const tc_coef_t & c = A.c;
@@ -641,52 +649,108 @@ __device__ __forceinline__ CorrectCol correct_col(const CorrectArgs & A, int i){
   C.fz = prefactor / c.hz;
   const double pf = 0.5 * gamma * dt * c.nu;          /* update_pressure.c:155-156 */
   C.qf = 2. * pf / c.jdxc[i];
-  C.xl = c.plx_l[i]; C.xu = c.plx_u[i]; C.xc = - C.xl - C.xu;
-  C.yl = c.ply[i]; C.yc = - C.yl - C.yl;
-  C.zl = c.plz[i]; C.zc = - C.zl - C.zl;
+  C.xl = c.plx_l[i]; C.xu = c.plx_u[i];
+  C.yl = c.ply[i];
+  C.zl = c.plz[i];
   return C;
 }
 
-/* one cell of the correction; pointers start at plane k-1 (kb0), 32-bit offsets inside the
- * 3-plane window.  CFL: also the cell's bound of decide_dt_adv on the corrected velocity. */
+/* one cell of the correction.  Every load of the cell is issued before its first store (the
+ * vector-memory counter counts loads and stores together, so a load issued after a store
+ * could be waited for only by draining both).  The fields are read and written through
+ * buffer resources of plane k and psi through one of planes k-1 .. k+1 based one row and one
+ * cell before the plane, so that every address is one per-lane offset (ob: the cell's byte
+ * offset in a plane of doubles, obr: in a plane of R) plus a uniform one.  CFL: also the
+ * cell's bound of decide_dt_adv on the corrected velocity. */
 template <typename R, bool CFL>
-__device__ __forceinline__ double correct_body(const CorrectArgs & A, const CorrectCol & C,
-    double * __restrict__ ux, double * __restrict__ uy, double * __restrict__ uz,
-    double * __restrict__ pp, const R * __restrict__ psi, const int i, const int j, const int k,
-    const long long kb0){
+__device__ __forceinline__ double correct_cell(const CorrectArgs & A, const CorrectCol & C,
+    const int i, const int j, const int k, const unsigned ob, const unsigned obr){
   const tc_geom_t & g = A.g;
-  double lmin = 1.;
+  const long long pl = g.plane;
   const int sy = g.px;
-  const int sz = (int)g.plane;
-  const int o = (i + TC_XOFF) + sy * j + sz;
+  const long long kk = (long long)k * pl;
+  const unsigned fb = (unsigned)pl * 8u;
+  const __amdgpu_buffer_rsrc_t rux = tck_rsrc(A.ux + kk, fb), ruy = tck_rsrc(A.uy + kk, fb),
+      ruz = tck_rsrc(A.uz + kk, fb), rp = tck_rsrc(A.p + kk, fb);
+  const unsigned sr = (unsigned)sizeof(R);
+  const __amdgpu_buffer_rsrc_t rps = tck_rsrc((const R *)A.psi + kk - pl - sy - 1,
+      (unsigned)(3 * pl + 2 * sy + 2) * sr);
+  const unsigned c0 = (unsigned)(pl + sy + 1) * sr;        /* the cell itself */
+  double lmin = 1.;
   const double small = 1.e-8;
-  const double ps0 = double(psi[o]);
-  if(A.mode & 1){
+  const bool vel = A.mode & 1, prs = A.mode & 2;
+  const bool ix = prs && A.f.implicit[0], iy = prs && A.f.implicit[1], iz = prs && A.f.implicit[2];
+  /* all loads unconditional (a launch that skips the velocity or the pressure part reads a
+   * field it does not use): no branch between them, no register cleared in between */
+  const double ps0 = double(tck_bld<R>(rps, obr, c0));
+  const double psxm = double(tck_bld<R>(rps, obr, c0 - sr));
+  const double psym = double(tck_bld<R>(rps, obr, c0 - (unsigned)sy * sr));
+  const double pszm = double(tck_bld<R>(rps, obr, c0 - (unsigned)pl * sr));
+  const double psxp = double(tck_bld<R>(rps, obr, c0 + sr));
+  const double psyp = double(tck_bld<R>(rps, obr, c0 + (unsigned)sy * sr));
+  const double pszp = double(tck_bld<R>(rps, obr, c0 + (unsigned)pl * sr));
+  const double u0 = tck_bld<double>(rux, ob, 0), v0 = tck_bld<double>(ruy, ob, 0);
+  const double w0 = tck_bld<double>(ruz, ob, 0), p0 = tck_bld<double>(rp, ob, 0);
+  /* halo images (src/halo.c:18-142: j = 0 from ny, j = ny+1 from 1, then the z planes over the
+   * full (x, y) extent, so corners are images of images): row nj of plane k, and plane nk */
+  const int nj = (j == 1) ? g.ny + 1 : (j == g.ny) ? 0 : -1;
+  const int nk = (k == 1) ? g.nz + 1 : (k == g.nz) ? 0 : -1;
+  const unsigned obj = ob + (unsigned)((nj - j) * sy) * 8u;
+  const long long kn = (long long)nk * pl;
+  double vx = 0., vy = 0., vz = 0., vp = 0.;
+  if(vel){
     if(i >= 2){ /* correct/ux.c:25-52 */
-      const double v = ux[o] - C.fx * (- double(psi[o - 1]) + ps0);
-      tck_store_halo(ux - kb0, g, i, j, k, v, false);
-      if(CFL) lmin = fmin(lmin, C.hx / (fabs(v) + small));
+      vx = u0 - C.fx * (- psxm + ps0);
+      tck_bst(rux, ob, vx);
+      if(CFL) lmin = fmin(lmin, C.hx / (fabs(vx) + small));
     }
     { /* correct/uy.c:25-52 */
-      const double v = uy[o] - C.fy * (- double(psi[o - sy]) + ps0);
-      tck_store_halo(uy - kb0, g, i, j, k, v, false);
-      if(CFL) lmin = fmin(lmin, C.hy / (fabs(v) + small));
+      vy = v0 - C.fy * (- psym + ps0);
+      tck_bst(ruy, ob, vy);
+      if(CFL) lmin = fmin(lmin, C.hy / (fabs(vy) + small));
     }
     { /* correct/uz.c:15-44 */
-      const double v = uz[o] - C.fz * (- double(psi[o - sz]) + ps0);
-      tck_store_halo(uz - kb0, g, i, j, k, v, false);
-      if(CFL) lmin = fmin(lmin, A.c.hz / (fabs(v) + small));
+      vz = w0 - C.fz * (- pszm + ps0);
+      tck_bst(ruz, ob, vz);
+      if(CFL) lmin = fmin(lmin, A.c.hz / (fabs(vz) + small));
     }
   }
-  if(A.mode & 2){ /* update_pressure.c:146-173 */
-    double v = pp[o] + ps0;
-    if(A.f.implicit[0])
-      v -= C.qf * (+ C.xl * double(psi[o - 1]) + C.xc * ps0 + C.xu * double(psi[o + 1]));
-    if(A.f.implicit[1])
-      v -= C.qf * (+ C.yl * double(psi[o - sy]) + C.yc * ps0 + C.yl * double(psi[o + sy]));
-    if(A.f.implicit[2])
-      v -= C.qf * (+ C.zl * double(psi[o - sz]) + C.zc * ps0 + C.zl * double(psi[o + sz]));
-    tck_store_halo(pp - kb0, g, i, j, k, v, true);
+  if(prs){ /* update_pressure.c:146-173 */
+    vp = p0 + ps0;
+    if(ix) vp -= C.qf * (+ C.xl * psxm + (- C.xl - C.xu) * ps0 + C.xu * psxp);
+    if(iy) vp -= C.qf * (+ C.yl * psym + (- C.yl - C.yl) * ps0 + C.yl * psyp);
+    if(iz) vp -= C.qf * (+ C.zl * pszm + (- C.zl - C.zl) * ps0 + C.zl * pszp);
+    tck_bst(rp, ob, vp);
+  }
+  if(nj >= 0){
+    if(vel){
+      if(i >= 2) tck_bsti(rux, obj, vx);
+      tck_bsti(ruy, obj, vy);
+      tck_bsti(ruz, obj, vz);
+    }
+    if(prs) tck_bsti(rp, obj, vp);
+  }
+  if(nk >= 0){
+    const __amdgpu_buffer_rsrc_t nux = tck_rsrc(A.ux + kn, fb), nuy = tck_rsrc(A.uy + kn, fb),
+        nuz = tck_rsrc(A.uz + kn, fb), np = tck_rsrc(A.p + kn, fb);
+    if(vel){
+      if(i >= 2){ tck_bsti(nux, ob, vx); if(nj >= 0) tck_bsti(nux, obj, vx); }
+      tck_bsti(nuy, ob, vy); if(nj >= 0) tck_bsti(nuy, obj, vy);
+      tck_bsti(nuz, ob, vz); if(nj >= 0) tck_bsti(nuz, obj, vz);
+    }
+    if(prs){ tck_bsti(np, ob, vp); if(nj >= 0) tck_bsti(np, obj, vp); }
+  }
+  /* the x wall copy of p (src/fluid/boundary/p.c): i = 0 from 1, i = nx + 1 from nx, with its
+   * own images */
+  if(prs && (i == 1 || i == g.nx)){
+    const unsigned dx = i == 1 ? 0u - 8u : 8u;
+    tck_bst(rp, ob + dx, vp);
+    if(nj >= 0) tck_bsti(rp, obj + dx, vp);
+    if(nk >= 0){
+      const __amdgpu_buffer_rsrc_t np = tck_rsrc(A.p + kn, fb);
+      tck_bsti(np, ob + dx, vp);
+      if(nj >= 0) tck_bsti(np, obj + dx, vp);
+    }
   }
   return lmin;
 }
@@ -706,21 +770,39 @@ __device__ __forceinline__ void dt_block_min(double v, unsigned long long * dtmi
   }
 }
 
+/* the per-column factors of a block's 64 columns, in LDS: read again for every plane rather
+ * than held in registers, which leaves the kernel at 8 waves per SIMD */
+#define TCK_CCOLS 9
 template <typename R, bool CFL>
 __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
+  __shared__ double colv[TCK_CCOLS][TCK_WAVE];
   int bx, by, bz;
   tile_of(A.tm, &bx, &by, &bz);
   const int i = 1 + bx * TCK_WAVE + threadIdx.x;
   const int j = 1 + by * blockDim.y + threadIdx.y;
   const int k0 = 1 + bz * A.tm.kb;
   const int k1 = min(k0 + A.tm.kb, A.g.nz + 1);
+  if(threadIdx.y == 0){
+    const CorrectCol C = correct_col(A, min(i, A.g.nx));
+    const double v[TCK_CCOLS] = {C.fx, C.fy, C.qf, C.xl, C.xu, C.yl, C.zl, C.hx, C.hy};
+    #pragma unroll
+    for(int q = 0; q < TCK_CCOLS; q++) colv[q][threadIdx.x] = v[q];
+  }
+  __syncthreads();
   double lmin = 1.;
   if(i <= A.g.nx && j <= A.g.ny){
-    const CorrectCol C = correct_col(A, i);
+    CorrectCol C;
+    C.fz = correct_col_fz(A);
+    /* the cell's offset within a plane (i <= nx + 1, j <= ny + 1: far below 2^32 bytes) */
+    const unsigned oc = (unsigned)((i + TC_XOFF) + A.g.px * j);
+    const unsigned ob = oc * 8u, obr = oc * (unsigned)sizeof(R);
     for(int k = k0; k < k1; k++){
-      const long long kb0 = (long long)(k - 1) * A.g.plane;
-      lmin = fmin(lmin, correct_body<R, CFL>(A, C, A.ux + kb0, A.uy + kb0, A.uz + kb0, A.p + kb0,
-          (const R *)A.psi + kb0, i, j, k, kb0));
+      /* keeps the column factors from being hoisted out of the loop into registers */
+      asm volatile("" ::: "memory");
+      const int x = threadIdx.x;
+      C.fx = colv[0][x]; C.fy = colv[1][x]; C.qf = colv[2][x]; C.xl = colv[3][x]; C.xu = colv[4][x];
+      C.yl = colv[5][x]; C.zl = colv[6][x]; C.hx = colv[7][x]; C.hy = colv[8][x];
+      lmin = fmin(lmin, correct_cell<R, CFL>(A, C, i, j, k, ob, obr));
     }
   }
   /* CFL bound of decide_dt_adv (decide_dt.c:32-80) on the corrected velocity */
@@ -846,6 +928,8 @@ int tck_launch_correct(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
     double * ux, double * uy, double * uz, double * p, const void * psi, const double * dt,
     int rkstep, int mode, unsigned long long * dtmin, int mixed, int kb, int ty, hipStream_t stream){
   if(check_shape(kb, ty)) return (int)hipErrorInvalidValue;
+  /* buffer resources span three planes of psi: their size must fit the 31-bit record count */
+  if((3 * g->plane + 2 * (long long)g->px + 2) * 8 >= (1LL << 31)) return (int)hipErrorInvalidValue;
   CorrectArgs A;
   A.mode = mode;
   A.g = *g; A.c = *c; A.f = *f;
