@@ -676,6 +676,54 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   const Q num = smX[2 * TI + ti] - tu * smX[ti] - tl * smX[TI + ti];
   const Q xl = fabs(double(den)) < DBL_EPSILON ? Q(0.) : num / den;
   if(!valid) return;
+  if constexpr(sizeof(Q) == 8){
+    /* FP64: every load of the pass (q1, the old field) before the first store, the results
+     * formed in place in q; then the stores.  A load issued after a store could be waited for
+     * only together with the store. */
+    double ulast = 0.;
+    if(sg == S - 1 && ucur) ulast = ucur[base + step * m];
+    #pragma unroll
+    for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
+      if(e0 < cnt){
+        Q qq[PS_PRE];
+        double uu[PS_PRE];
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          const int row = row0 + e0 + e < m ? row0 + e0 + e : m - 1;
+          qq[e] = Q(q1[ts * row]);
+          uu[e] = ucur ? ucur[base + step * row] : 0.;
+        }
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          const Q x = q[e0 + e] + xl * qq[e];
+          q[e0 + e] = ucur ? Q(uu[e] + double(x)) : x;
+        }
+      }
+    }
+    #pragma unroll
+    for(int e = 0; e < MAXM; e++){
+      if(e < cnt){
+        const int row = row0 + e;
+        if(ucur){
+          const int jj = A.dir == 1 ? 1 + row : other;
+          const int kk = A.dir == 1 ? other : 1 + row;
+          tck_store_halo(uout, g, i, jj, kk, double(q[e]), false);
+        }else{
+          tck_st(&d[base + step * row], D(q[e]));
+        }
+      }
+    }
+    if(sg == S - 1){
+      if(ucur){
+        const int jj = A.dir == 1 ? n : other;
+        const int kk = A.dir == 1 ? other : n;
+        tck_store_halo(uout, g, i, jj, kk, ulast + double(xl), false);
+      }else{
+        tck_st(&d[base + step * m], D(xl));
+      }
+    }
+    return;
+  }
   /* q1 and the old field, PS_PRE rows at a time */
   #pragma unroll
   for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
