@@ -8,13 +8,10 @@
  * periodic like the flow.
  *
  * Activations are channels-last BF16 rows act[v][C], v = x + NX (y + NY (z + NZ b)) for
- * sample b of a batch.  The convolution
- * is an implicit GEMM on the matrix cores: for 16 consecutive cells of a row and 16 output
- * channels a wavefront accumulates, over the 27 taps and the input channels in groups of 16,
- * v_mfma_f32_16x16x16_bf16 products in FP32 (lane l holds A row l % 16 = cell, B column
- * l % 16 = output channel, both over input channels 4 (l / 16) .. +3; the accumulator holds
- * output rows 4 (l / 16) .. +3 of column l % 16).  Weights are W[tap][co][ci] BF16 with
- * FP32 master copies; biases FP32.
+ * sample b of a batch.  The convolution is an implicit GEMM on the matrix cores
+ * (v_mfma_f32_16x16x32_bf16, FP32 accumulation) over activation tiles staged in LDS with
+ * their halo; the weight gradient uses v_mfma_f32_16x16x16_bf16.  Weights are W[tap][co][ci]
+ * BF16 with FP32 master copies; biases FP32.
  *
  * Training: the backward data pass is the same convolution with the kernel flipped and its
  * channel roles exchanged (the adjoint of zero padding is zero padding, of periodic wrapping
@@ -53,9 +50,21 @@ __device__ __forceinline__ long long cnn_tap(const CnnGrid & G, int x, int y, in
 
 /* ---------------------------------------------------------------------------------------
  * convolution: out[v][co] = bias[co] + sum_tap sum_ci W[tap][co][ci] in[v + tap][ci]
- *   CI input channels (multiple of 16), CO output channels (multiple of 16) of which the
- *   first co_store are written; relu: out = max(0, .) stored BF16 (next layer's input),
- *   otherwise FP32.  One wavefront per 16 cells x all CO; 4 waves per block.
+ *   CI input channels (16 or 32), CO output channels (16 or 32) of which the first co_store
+ *   are written; relu: out = max(0, .) stored BF16 (next layer's input), otherwise FP32.
+ *
+ * Implicit GEMM on v_mfma_f32_16x16x32_bf16: A = 16 cells of an x row by 32 K entries, B = 32
+ * K entries by 16 output channels.  K is the 32 input channels of one tap (CI = 32), or the
+ * 16 input channels of two taps (CI = 16, the 27th tap paired with zeros).  Lane l holds the
+ * eight K entries 8 (l / 16) .. +7 of A row l % 16 and of B column l % 16 (the same K order on
+ * both sides, which is all the product needs); the accumulator holds rows (cells)
+ * 4 (l / 16) .. +3 of column l % 16.
+ *
+ * A block stages the kernel once (rows of CI BF16 padded by 16 bytes, so that the 16 lanes
+ * reading one K group hit distinct banks) and loops over tiles of 16 x CNN_TY x CNN_TZ cells
+ * of one sample: each tile's activations with a one-cell halo (zero outside the walls in x,
+ * periodic images in y and z) are staged in LDS with the same padded rows, and every tap of
+ * every cell is then read from LDS.  Each wave takes CNN_RPW rows of the tile.
  * ------------------------------------------------------------------------------------- */
 struct ConvArgs {
   CnnGrid G;
@@ -69,60 +78,114 @@ struct ConvArgs {
 };
 
 #define CNN_CONV_THREADS 512
+#define CNN_TY 4
+#define CNN_TZ 4
+#define CNN_HX 18                             /* halo'd tile: 16 + 2 cells in x */
+#define CNN_HCELLS (CNN_HX * (CNN_TY + 2) * (CNN_TZ + 2))
+#define CNN_RPW ((CNN_TY * CNN_TZ) / (CNN_CONV_THREADS / TCK_WAVE))   /* rows per wave */
 
-/* The block stages the whole kernel W (27 CO CI BF16, at most 54 KiB) in LDS once and its
- * waves then loop over 16-cell segments, so the B operands come from LDS and only the
- * activations are read from memory. */
+typedef __bf16 cnn_b8 __attribute__((ext_vector_type(8)));
+
+/* LDS bytes of the convolution of CI -> CO channels: the kernel and one halo'd tile */
+__host__ __device__ constexpr int cnn_conv_lds(int ci, int co){
+  return (27 * co + CNN_HCELLS) * (2 * ci + 16);
+}
+
+__device__ __forceinline__ int cnn_wrap(int a, int n){
+  a %= n;
+  return a < 0 ? a + n : a;
+}
+
 template <int CI, int CO>
 __global__ void __launch_bounds__(CNN_CONV_THREADS) tck_cnn_conv(const ConvArgs A){
-  constexpr int NW = 27 * CO * CI;
-  __shared__ __align__(16) unsigned short wl[NW];
-  for(int e = 8 * threadIdx.x; e < NW; e += 8 * CNN_CONV_THREADS) *(uint4 *)(wl + e) = *(const uint4 *)(A.w + e);
-  __syncthreads();
+  constexpr int RS = 2 * CI + 16;                 /* padded row: bytes per cell or per co */
+  constexpr int NT = CO / 16;                     /* 16-channel output blocks */
+  constexpr int NK = CI == 32 ? 27 : 14;          /* MFMA steps over the taps */
+  extern __shared__ __align__(16) unsigned char cnn_smem[];
+  unsigned char * wl = cnn_smem;                  /* [27][CO] rows */
+  unsigned char * al = cnn_smem + 27 * CO * RS;   /* [HZ][HY][HX] rows */
+  /* the kernel: 16-byte pieces, CI / 8 per row */
+  for(int e = threadIdx.x; e < 27 * CO * (CI / 8); e += CNN_CONV_THREADS){
+    const int row = e / (CI / 8), pc = e % (CI / 8);
+    *(uint4 *)(wl + row * RS + 16 * pc) = *(const uint4 *)(A.w + (long long)row * CI + 8 * pc);
+  }
   const int lane = threadIdx.x % TCK_WAVE;
-  const int wpb = CNN_CONV_THREADS / TCK_WAVE;
-  const int rows = (A.G.nx + 15) / 16;                 /* 16-cell segments per x row */
-  const long long nseg = (long long)rows * A.G.ny * A.G.nz * A.G.nb;
-  const int kq = 4 * (lane / 16);                      /* this lane's 4 K entries */
-  for(long long wave = (long long)blockIdx.x * wpb + threadIdx.x / TCK_WAVE; wave < nseg; wave += (long long)gridDim.x * wpb){
-    const int seg = (int)(wave % rows);
-    const long long yzb = wave / rows;
-    const int y = (int)(yzb % A.G.ny);
-    const int z = (int)((yzb / A.G.ny) % A.G.nz);
-    const int b = (int)(yzb / ((long long)A.G.ny * A.G.nz));
-    const int x = seg * 16 + lane % 16;                /* A row: this lane's cell */
-    cnn_f4 acc[CO / 16];
+  const int wv = threadIdx.x / TCK_WAVE;
+  const int kg = lane / 16;                       /* this lane's K group */
+  const int cl = lane % 16;                       /* A row (cell in x) / B column (channel) */
+  const CnnGrid & G = A.G;
+  const int ntx = (G.nx + 15) / 16, nty = (G.ny + CNN_TY - 1) / CNN_TY, ntz = (G.nz + CNN_TZ - 1) / CNN_TZ;
+  const long long ntiles = (long long)ntx * nty * ntz * G.nb;
+  /* CI = 16: K group kg reads tap 2 s + (kg >> 1), channels 8 (kg & 1) .. +7 */
+  const int ktap = CI == 32 ? 0 : (kg >> 1);
+  const int kbyte = CI == 32 ? 16 * kg : 16 * (kg & 1);
+  for(long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x){
+    const int tx = (int)(tile % ntx);
+    long long r = tile / ntx;
+    const int ty = (int)(r % nty); r /= nty;
+    const int tz = (int)(r % ntz);
+    const int b = (int)(r / ntz);
+    const int x0 = 16 * tx, y0 = CNN_TY * ty, z0 = CNN_TZ * tz;
+    __syncthreads();                              /* the previous tile's reads are done */
+    for(int e = threadIdx.x; e < CNN_HCELLS * (CI / 8); e += CNN_CONV_THREADS){
+      const int c = e / (CI / 8), pc = e % (CI / 8);
+      const int hx = c % CNN_HX, hy = (c / CNN_HX) % (CNN_TY + 2), hz = c / (CNN_HX * (CNN_TY + 2));
+      const int xx = x0 + hx - 1;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if(xx >= 0 && xx < G.nx){
+        const int yy = cnn_wrap(y0 + hy - 1, G.ny), zz = cnn_wrap(z0 + hz - 1, G.nz);
+        const long long src = (long long)xx + (long long)G.nx * ((long long)yy + (long long)G.ny * ((long long)zz + (long long)G.nz * b));
+        v = *(const uint4 *)(A.in + src * CI + 8 * pc);
+      }
+      *(uint4 *)(al + c * RS + 16 * pc) = v;
+    }
+    __syncthreads();
+    cnn_f4 acc[CNN_RPW][NT];
     #pragma unroll
-    for(int t = 0; t < CO / 16; t++) acc[t] = cnn_f4{0.f, 0.f, 0.f, 0.f};
-    #pragma unroll 3
-    for(int tap = 0; tap < 27; tap++){
-      const long long src = x < A.G.nx ? cnn_tap(A.G, x, y, z, b, tap) : -1;
+    for(int q = 0; q < CNN_RPW; q++)
       #pragma unroll
-      for(int c0 = 0; c0 < CI; c0 += 16){
-        /* 4 consecutive BF16 channels: one 8-byte load (rows are 32-byte aligned) */
-        cnn_s4 a = {0, 0, 0, 0};
-        if(src >= 0) a = *(const cnn_s4 *)(A.in + src * CI + c0 + kq);
+      for(int t = 0; t < NT; t++) acc[q][t] = cnn_f4{0.f, 0.f, 0.f, 0.f};
+    for(int s = 0; s < NK; s++){
+      const int tap = CI == 32 ? s : 2 * s + ktap;
+      const bool live = tap < 27;
+      const int tp = live ? tap : 26;
+      const int dx = tp % 3, dy = (tp / 3) % 3, dz = tp / 9;   /* halo offsets 0..2 */
+      cnn_b8 bw[NT];
+      #pragma unroll
+      for(int t = 0; t < NT; t++){
+        bw[t] = *(const cnn_b8 *)(wl + (tp * CO + 16 * t + cl) * RS + kbyte);
+        if(!live) bw[t] = cnn_b8{};
+      }
+      #pragma unroll
+      for(int q = 0; q < CNN_RPW; q++){
+        const int row = wv * CNN_RPW + q;
+        const int ry = row % CNN_TY, rz = row / CNN_TY;
+        const int c = (cl + dx) + CNN_HX * ((ry + dy) + (CNN_TY + 2) * (rz + dz));
+        const cnn_b8 av = *(const cnn_b8 *)(al + c * RS + kbyte);
         #pragma unroll
-        for(int t = 0; t < CO / 16; t++){
-          const cnn_s4 wq = *(const cnn_s4 *)(wl + (tap * CO + t * 16 + lane % 16) * CI + c0 + kq);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wq, acc[t], 0, 0, 0);
-        }
+        for(int t = 0; t < NT; t++) acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[t], acc[q][t], 0, 0, 0);
       }
     }
-    /* accumulator: rows (cells) kq .. kq+3 of column (output channel) lane % 16 */
+    /* accumulator: cells 4 kg .. +3 of the row, output channel 16 t + cl */
     #pragma unroll
-    for(int t = 0; t < CO / 16; t++){
-      const int co = t * 16 + lane % 16;
-      if(co >= A.co_store) continue;
-      const float bv = A.bias ? A.bias[co] : 0.f;
+    for(int q = 0; q < CNN_RPW; q++){
+      const int row = wv * CNN_RPW + q;
+      const int y = y0 + row % CNN_TY, z = z0 + row / CNN_TY;
+      if(y >= G.ny || z >= G.nz) continue;
       #pragma unroll
-      for(int q = 0; q < 4; q++){
-        const int xc = seg * 16 + kq + q;
-        if(xc >= A.G.nx) continue;
-        const long long v = (long long)xc + (long long)A.G.nx * ((long long)y + (long long)A.G.ny * ((long long)z + (long long)A.G.nz * b));
-        const float val = acc[t][q] + bv;
-        if(A.relu) A.out_bf[v * A.co_store + co] = cnn_bf16(val > 0.f ? val : 0.f);
-        else A.out_f[v * A.co_store + co] = val;
+      for(int t = 0; t < NT; t++){
+        const int co = 16 * t + cl;
+        if(co >= A.co_store) continue;
+        const float bv = A.bias ? A.bias[co] : 0.f;
+        #pragma unroll
+        for(int u = 0; u < 4; u++){
+          const int xc = x0 + 4 * kg + u;
+          if(xc >= G.nx) continue;
+          const long long v = (long long)xc + (long long)G.nx * ((long long)y + (long long)G.ny * ((long long)z + (long long)G.nz * b));
+          const float val = acc[q][t][u] + bv;
+          if(A.relu) A.out_bf[v * A.co_store + co] = cnn_bf16(val > 0.f ? val : 0.f);
+          else A.out_f[v * A.co_store + co] = val;
+        }
       }
     }
   }
@@ -343,6 +406,19 @@ __global__ void tck_cnn_scatter(const ScatterArgs A){
 }
 
 /* ------------------------------------------------------------------------------------- */
+/* a convolution launch; the first one of an instantiation raises its dynamic LDS limit */
+template <int CI, int CO>
+static int cnn_conv_launch(const ConvArgs & A, dim3 grd, int lds, hipStream_t stream){
+  static int ready = 0;
+  if(!ready){
+    const hipError_t e = hipFuncSetAttribute((const void *)tck_cnn_conv<CI, CO>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if(e != hipSuccess) return (int)e;
+    ready = 1;
+  }
+  hipLaunchKernelGGL((tck_cnn_conv<CI, CO>), grd, dim3(CNN_CONV_THREADS), lds, stream, A);
+  return 0;
+}
+
 static int g_cnn_ncu = 0;
 
 static int cnn_ncu(void){
@@ -363,20 +439,21 @@ int tck_cnn_launch_conv(int nx, int ny, int nz, int nb, const unsigned short * i
   A.G.nx = nx; A.G.ny = ny; A.G.nz = nz; A.G.nb = nb; A.G.nv = (long long)nx * ny * nz * nb;
   A.in = in; A.w = w; A.bias = bias; A.out_bf = out_bf; A.out_f = out_f;
   A.co_store = co_store; A.relu = relu;
-  /* as many blocks as the LDS holds per CU (the staged kernel takes 27 co ci 2 bytes), at
-   * most four, each looping over segments */
-  const long long nseg = (long long)((nx + 15) / 16) * ny * nz * nb;
-  const int wpb = CNN_CONV_THREADS / TCK_WAVE;
-  long long per_cu = (160LL * 1024) / (27LL * co * ci * 2);
+  /* persistent blocks, as many as the LDS holds per CU (the kernel and a halo'd tile) */
+  const long long ntiles = (long long)((nx + 15) / 16) * ((ny + CNN_TY - 1) / CNN_TY) * ((nz + CNN_TZ - 1) / CNN_TZ) * nb;
+  const int lds = cnn_conv_lds(ci, co);
+  long long per_cu = (160LL * 1024) / lds;
   per_cu = per_cu < 1 ? 1 : (per_cu > 4 ? 4 : per_cu);
-  long long nblk = (nseg + wpb - 1) / wpb;
+  long long nblk = ntiles;
   if(nblk > per_cu * cnn_ncu()) nblk = per_cu * cnn_ncu();
-  const dim3 grd((unsigned)nblk), blk(CNN_CONV_THREADS);
-  if(ci == 16 && co == 32) hipLaunchKernelGGL((tck_cnn_conv<16, 32>), grd, blk, 0, stream, A);
-  else if(ci == 32 && co == 32) hipLaunchKernelGGL((tck_cnn_conv<32, 32>), grd, blk, 0, stream, A);
-  else if(ci == 32 && co == 16) hipLaunchKernelGGL((tck_cnn_conv<32, 16>), grd, blk, 0, stream, A);
-  else if(ci == 16 && co == 16) hipLaunchKernelGGL((tck_cnn_conv<16, 16>), grd, blk, 0, stream, A);
+  const dim3 grd((unsigned)nblk);
+  int rc;
+  if(ci == 16 && co == 32) rc = cnn_conv_launch<16, 32>(A, grd, lds, stream);
+  else if(ci == 32 && co == 32) rc = cnn_conv_launch<32, 32>(A, grd, lds, stream);
+  else if(ci == 32 && co == 16) rc = cnn_conv_launch<32, 16>(A, grd, lds, stream);
+  else if(ci == 16 && co == 16) rc = cnn_conv_launch<16, 16>(A, grd, lds, stream);
   else return (int)hipErrorInvalidValue;
+  if(rc) return rc;
   TCK_CHECK_LAUNCH();
   return 0;
 }
