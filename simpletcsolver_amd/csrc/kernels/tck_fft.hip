@@ -371,6 +371,9 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
 #ifndef TCK_DIVY_PF
 #define TCK_DIVY_PF 1
 #endif
+#ifndef TCK_DIVY_KRUN
+#define TCK_DIVY_KRUN 1
+#endif
 
 template <typename T, int PF, int EPTK, bool W>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs A){
@@ -413,12 +416,30 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
    * are loaded together */
   constexpr int EPT2 = 2 * EPTK;
   constexpr int G = 2;
+#if TCK_DIVY_KRUN
+  /* each block walks a contiguous run of planes of one column of tiles, so that the uz
+   * plane a tile reads as its upper face is read again as the next tile's lower face soon
+   * after, from cache; tile index tile = tx + ntx (k - 1) as in the plain order */
+  const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int c0 = blockIdx.x * per, c1 = min(c0 + per, ntiles);
+  #define DV_TILE(c) (((c) / g.nz) + ntx * ((c) % g.nz))
+  int cc = c0;
+  if(cc < c1){
+    #pragma unroll
+    for(int q = 0; q < PF; q++){ DV_LOAD(DV_TILE(cc), q, pre[q], pok[q]); }
+  }
+  for(; cc < c1; cc++){
+    const int tile = DV_TILE(cc);
+    const int tnext = cc + 1 < c1 ? DV_TILE(cc + 1) : ntiles;
+#else
   int tile = blockIdx.x;
   if(tile < ntiles){
     #pragma unroll
     for(int q = 0; q < PF; q++){ DV_LOAD(tile, q, pre[q], pok[q]); }
   }
   for(; tile < ntiles; tile += gridDim.x){
+    const int tnext = tile + (int)gridDim.x;
+#endif
     /* the element offsets are recomputed in every tile rather than held across the passes */
     t0 = t0i; r0 = r0i;
     asm volatile("" : "+v"(t0), "+v"(r0));
@@ -455,9 +476,9 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
       for(int u = 0; u < G; u++) if(q0 + u < EPT2) put(q0 + u, P[u], ok[u]);
     }
     fft_lds_barrier();
-    if(tile + (int)gridDim.x < ntiles){
+    if(tnext < ntiles){
       #pragma unroll
-      for(int q = 0; q < PF; q++){ DV_LOAD(tile + (int)gridDim.x, q, pre[q], pok[q]); }
+      for(int q = 0; q < PF; q++){ DV_LOAD(tnext, q, pre[q], pok[q]); }
     }
     lds_fft<T, EPTK, W>(sm, A.ld, ti, A.P, twM, T(-1.));
     for(int e = threadIdx.x; e < ti * (M + 1); e += nt){
@@ -477,6 +498,9 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
     fft_lds_barrier();
   }
   #undef DV_LOAD
+#if TCK_DIVY_KRUN
+  #undef DV_TILE
+#endif
 }
 
 /* ------------------------------------------------------------------------------------- */
