@@ -660,11 +660,14 @@ __device__ __forceinline__ CorrectCol correct_col(const CorrectArgs & A, int i){
  * could be waited for only by draining both).  The fields are read and written through
  * buffer resources of plane k and psi through one of planes k-1 .. k+1 based one row and one
  * cell before the plane, so that every address is one per-lane offset (ob: the cell's byte
- * offset in a plane of doubles, obr: in a plane of R) plus a uniform one.  CFL: also the
- * cell's bound of decide_dt_adv on the corrected velocity. */
+ * offset in a plane of doubles, obr: in a plane of R) plus a uniform one.  psi of the cell
+ * at planes k and k - 1 comes in from the previous plane (ps0, pszm); the one at k + 1 goes
+ * out for the next (pszp_out).  CFL: also the cell's bound of decide_dt_adv on the
+ * corrected velocity. */
 template <typename R, bool CFL>
 __device__ __forceinline__ double correct_cell(const CorrectArgs & A, const CorrectCol & C,
-    const int i, const int j, const int k, const unsigned ob, const unsigned obr){
+    const int i, const int j, const int k, const unsigned ob, const unsigned obr,
+    const double ps0, const double pszm, double & pszp_out){
   const tc_geom_t & g = A.g;
   const long long pl = g.plane;
   const int sy = g.px;
@@ -682,13 +685,12 @@ __device__ __forceinline__ double correct_cell(const CorrectArgs & A, const Corr
   const bool ix = prs && A.f.implicit[0], iy = prs && A.f.implicit[1], iz = prs && A.f.implicit[2];
   /* all loads unconditional (a launch that skips the velocity or the pressure part reads a
    * field it does not use): no branch between them, no register cleared in between */
-  const double ps0 = double(tck_bld<R>(rps, obr, c0));
   const double psxm = double(tck_bld<R>(rps, obr, c0 - sr));
   const double psym = double(tck_bld<R>(rps, obr, c0 - (unsigned)sy * sr));
-  const double pszm = double(tck_bld<R>(rps, obr, c0 - (unsigned)pl * sr));
   const double psxp = double(tck_bld<R>(rps, obr, c0 + sr));
   const double psyp = double(tck_bld<R>(rps, obr, c0 + (unsigned)sy * sr));
   const double pszp = double(tck_bld<R>(rps, obr, c0 + (unsigned)pl * sr));
+  pszp_out = pszp;
   const double u0 = tck_bld<double>(rux, ob, 0), v0 = tck_bld<double>(ruy, ob, 0);
   const double w0 = tck_bld<double>(ruz, ob, 0), p0 = tck_bld<double>(rp, ob, 0);
   /* halo images (src/halo.c:18-142: j = 0 from ny, j = ny+1 from 1, then the z planes over the
@@ -796,13 +798,20 @@ __global__ void __launch_bounds__(TCK_BLOCK) tck_correct(const CorrectArgs A){
     /* the cell's offset within a plane (i <= nx + 1, j <= ny + 1: far below 2^32 bytes) */
     const unsigned oc = (unsigned)((i + TC_XOFF) + A.g.px * j);
     const unsigned ob = oc * 8u, obr = oc * (unsigned)sizeof(R);
+    /* psi of the cell at planes k - 1 and k, carried from plane to plane: each plane of psi
+     * is read from memory once per column of cells */
+    const R * psc = (const R *)A.psi + oc;
+    double pszm = double(psc[(long long)(k0 - 1) * A.g.plane]), ps0 = double(psc[(long long)k0 * A.g.plane]);
     for(int k = k0; k < k1; k++){
       /* keeps the column factors from being hoisted out of the loop into registers */
       asm volatile("" ::: "memory");
       const int x = threadIdx.x;
       C.fx = colv[0][x]; C.fy = colv[1][x]; C.qf = colv[2][x]; C.xl = colv[3][x]; C.xu = colv[4][x];
       C.yl = colv[5][x]; C.zl = colv[6][x]; C.hx = colv[7][x]; C.hy = colv[8][x];
-      lmin = fmin(lmin, correct_cell<R, CFL>(A, C, i, j, k, ob, obr));
+      double pszp;
+      lmin = fmin(lmin, correct_cell<R, CFL>(A, C, i, j, k, ob, obr, ps0, pszm, pszp));
+      pszm = ps0;
+      ps0 = pszp;
     }
   }
   /* CFL bound of decide_dt_adv (decide_dt.c:32-80) on the corrected velocity */
