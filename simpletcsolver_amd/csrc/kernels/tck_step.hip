@@ -86,7 +86,7 @@ __device__ __forceinline__ double own_value(const FT * staged, int of, const dou
 
 /* body with restrict-qualified pointers, so that the loads of each field section are free
  * to move ahead of the stores of the previous one */
-template <typename R, typename PT, typename FT, int TW, int CM>
+template <typename R, typename PT, typename FT, int TW, int CM, int IMP = -1>
 __device__ __forceinline__ void predict_body(const PredictArgs & A,
     const FT * __restrict__ Fux_m, const FT * __restrict__ Fux_0, const FT * __restrict__ Fux_p,
     const FT * __restrict__ Fuy_m, const FT * __restrict__ Fuy_0, const FT * __restrict__ Fuy_p,
@@ -114,7 +114,12 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
   const R ca = c_rka[s] * dt;
   const R cb = c_rkb[s] * dt;
   const R cg = (c_rka[s] + c_rkb[s]) * dt;
-  const bool ix = A.f.implicit[0], iy = A.f.implicit[1], iz = A.f.implicit[2];
+  /* IMP >= 0: the implicit directions as compile-time bits (a line solve follows, so no
+   * final store); IMP < 0: from the flags */
+  const bool ix = IMP >= 0 ? (IMP & 1) != 0 : A.f.implicit[0];
+  const bool iy = IMP >= 0 ? (IMP & 2) != 0 : A.f.implicit[1];
+  const bool iz = IMP >= 0 ? (IMP & 4) != 0 : A.f.implicit[2];
+  const bool fin = IMP >= 0 ? false : A.final_here != 0;
 
   /* ---------------- ux, faces i = 2..nx (predict/ux.c) ---------------- */
   if(i >= 2){
@@ -190,7 +195,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbx) b = sbv ? sbv[0] : sbx[o];
     const R delta = (sbx ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sax) tck_st(&sax[o], a);
-    if(A.final_here) tck_store_halo(nxo - kb0, g, i, j, k, own_value(Fux_0, of, A.ux, kb0 + o) + (double)delta, false);
+    if(fin) tck_store_halo(nxo - kb0, g, i, j, k, own_value(Fux_0, of, A.ux, kb0 + o) + (double)delta, false);
     else tck_st(&dxo[o], delta);
   }
 
@@ -279,7 +284,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sby) b = sbv ? sbv[1] : sby[o];
     const R delta = (sby ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(say) tck_st(&say[o], a);
-    if(A.final_here) tck_store_halo(nyo - kb0, g, i, j, k, own_value(Fuy_0, of, A.uy, kb0 + o) + (double)delta, false);
+    if(fin) tck_store_halo(nyo - kb0, g, i, j, k, own_value(Fuy_0, of, A.uy, kb0 + o) + (double)delta, false);
     else tck_st(&dyo[o], delta);
   }
 
@@ -348,7 +353,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbz) b = sbv ? sbv[2] : sbz[o];
     const R delta = (sbz ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(saz) tck_st(&saz[o], a);
-    if(A.final_here) tck_store_halo(nzo - kb0, g, i, j, k, own_value(Fuz_0, of, A.uz, kb0 + o) + (double)delta, false);
+    if(fin) tck_store_halo(nzo - kb0, g, i, j, k, own_value(Fuz_0, of, A.uz, kb0 + o) + (double)delta, false);
     else tck_st(&dzo[o], delta);
   }
 
@@ -398,7 +403,7 @@ __device__ __forceinline__ void predict_body(const PredictArgs & A,
     if(sbt) b = sbv ? sbv[3] : sbt[o];
     const R delta = (sbt ? + ca * a + cb * b : + ca * a) + cg * gg;
     if(sat) tck_st(&sat[o], a);
-    if(A.final_here) tck_store_halo(nto - kb0, g, i, j, k, own_value(Ft_0, of, A.t, kb0 + o) + (double)delta, false);
+    if(fin) tck_store_halo(nto - kb0, g, i, j, k, own_value(Ft_0, of, A.t, kb0 + o) + (double)delta, false);
     else tck_st(&dto[o], delta);
   }
 }
@@ -478,7 +483,7 @@ __device__ __forceinline__ void tck_lds_barrier(){
 /* SRC: RK sources of the previous substep are read (substeps 1 and 2); CM: closure
  * corrections are added.  Each combination is its own instantiation, so that the registers of
  * what a launch does not read are not held. */
-template <typename R, bool SRC, bool CM>
+template <typename R, bool SRC, bool CM, int IMP>
 __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) tck_predict_w(const PredictArgs A){
   __shared__ R smt[T_NPRED * TCK_WTW];
   /* ux uy uz T: a ring of three planes; p: two planes; in the arithmetic type (the stencils
@@ -582,7 +587,7 @@ __global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(
     #define RBT(ptr) ((ptr) ? (R *)(ptr) + kb0 : nullptr)
     #define RBC(ptr) ((ptr) ? (const R *)(ptr) + kb0 : nullptr)
     if(inside)
-    predict_body<R, R, R, TCK_WTW, CM ? 1 : 0>(A, wf[0][sm], wf[0][s0], wf[0][sp], wf[1][sm], wf[1][s0], wf[1][sp],
+    predict_body<R, R, R, TCK_WTW, CM ? 1 : 0, IMP>(A, wf[0][sm], wf[0][s0], wf[0][sp], wf[1][sm], wf[1][s0], wf[1][sp],
         wf[2][sm], wf[2][s0], wf[2][sp], wf[ft][sm], wf[ft][s0], wf[ft][sp],
         wq[pm], wq[1 - pm], of, TCK_WTW,
         RBT(A.dx), RBT(A.dy), RBT(A.dz), RBT(A.dtt),
@@ -918,12 +923,16 @@ int tck_launch_predict(const tc_geom_t * g, const tc_coef_t * c, const tc_flags_
   if(window){
     const bool src = A.sbx || A.sby || A.sbz || A.sbt;
     const bool cm = A.cx || A.cy || A.cz || A.ct;
-    #define TCK_PW(RT) \
-      if(src){ if(cm) hipLaunchKernelGGL((tck_predict_w<RT, true, true>), grd, blk, 0, stream, A); \
-               else hipLaunchKernelGGL((tck_predict_w<RT, true, false>), grd, blk, 0, stream, A); } \
-      else   { if(cm) hipLaunchKernelGGL((tck_predict_w<RT, false, true>), grd, blk, 0, stream, A); \
-               else hipLaunchKernelGGL((tck_predict_w<RT, false, false>), grd, blk, 0, stream, A); }
-    if(mixed){ TCK_PW(float) } else { TCK_PW(double) }
+    /* the all-implicit step (the benchmark's) has its own instantiations: no flag selects,
+     * no final-store path, fewer scalar registers */
+    const bool all3 = !final_here && f->implicit[0] && f->implicit[1] && f->implicit[2];
+    #define TCK_PW(RT, IM) \
+      if(src){ if(cm) hipLaunchKernelGGL((tck_predict_w<RT, true, true, IM>), grd, blk, 0, stream, A); \
+               else hipLaunchKernelGGL((tck_predict_w<RT, true, false, IM>), grd, blk, 0, stream, A); } \
+      else   { if(cm) hipLaunchKernelGGL((tck_predict_w<RT, false, true, IM>), grd, blk, 0, stream, A); \
+               else hipLaunchKernelGGL((tck_predict_w<RT, false, false, IM>), grd, blk, 0, stream, A); }
+    if(mixed){ if(all3){ TCK_PW(float, 7) } else { TCK_PW(float, -1) } }
+    else { if(all3){ TCK_PW(double, 7) } else { TCK_PW(double, -1) } }
     #undef TCK_PW
   }else{
     if(mixed) hipLaunchKernelGGL(tck_predict<float>, grd, blk, 0, stream, A);
