@@ -45,7 +45,18 @@ struct FftPlan {
   int M;
   int nst;
   int rad[FFT_MAXST];
+  /* per stage: multipliers that divide by M / rad and by the stage's span Ns with one high
+   * multiply (floor(2^32 / d) + 1, exact for dividends below 2^16; 0 for d = 1) */
+  unsigned mper[FFT_MAXST], mns[FFT_MAXST];
 };
+
+/* log2 of a tile width (tile widths are powers of two, at most 64) */
+__device__ __forceinline__ int fft_lg(int x){ return __builtin_ctz((unsigned)x); }
+
+/* x / d for the multiplier m of d (see FftPlan) */
+__device__ __forceinline__ unsigned fft_div(unsigned x, unsigned m){
+  return m ? __umulhi(x, m) : x;
+}
 
 template <typename V> __device__ __forceinline__ V cmul(V a, V b){
   V r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
@@ -198,7 +209,7 @@ __host__ __device__ __forceinline__ int fpad(int n){ return n + (n >> 3); }
 
 template <int R, typename T, int EPT = FFT_EPT>
 __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, int ld, int nb, int Ns,
-    const typename cvec<T>::type * __restrict__ tw, T s){
+    const typename cvec<T>::type * __restrict__ tw, T s, unsigned mper, unsigned mns){
   typedef typename cvec<T>::type V;
   constexpr int BPT = (EPT + R - 1) / R;
   const int nt = blockDim.x;
@@ -211,9 +222,9 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
   for(int q = 0; q < BPT; q++){
     const int bf = tid + q * nt;
     if(bf < nbf){
-      const int b = bf / per;
+      const int b = (int)fft_div((unsigned)bf, mper);
       const int jj = bf - b * per;
-      const int jm = jj % Ns;
+      const int jm = jj - (int)fft_div((unsigned)jj, mns) * Ns;
       const V * src = buf + b * ld;
       #pragma unroll
       for(int r = 0; r < R; r++){
@@ -233,9 +244,9 @@ __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, i
   for(int q = 0; q < BPT; q++){
     const int bf = tid + q * nt;
     if(bf < nbf){
-      const int b = bf / per;
+      const int b = (int)fft_div((unsigned)bf, mper);
       const int jj = bf - b * per;
-      const int jm = jj % Ns;
+      const int jm = jj - (int)fft_div((unsigned)jj, mns) * Ns;
       V * dst = buf + b * ld;
       const int d0 = (jj - jm) * R + jm;
       #pragma unroll
@@ -255,12 +266,13 @@ __device__ void lds_fft(typename cvec<T>::type * buf, int ld, int nb, const FftP
   int Ns = 1;
   for(int st = 0; st < P.nst; st++){
     const int R = P.rad[st];
-    if(R == 8) fft_stage<8, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(R == 4) fft_stage<4, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(R == 3) fft_stage<3, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(W && R == 5) fft_stage<5, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(W && R == 7) fft_stage<7, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
-    else if(R == 2) fft_stage<2, T, EPT>(buf, P.M, ld, nb, Ns, tw, s);
+    const unsigned mp = P.mper[st], mn = P.mns[st];
+    if(R == 8) fft_stage<8, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
+    else if(R == 4) fft_stage<4, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
+    else if(R == 3) fft_stage<3, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
+    else if(W && R == 5) fft_stage<5, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
+    else if(W && R == 7) fft_stage<7, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
+    else if(R == 2) fft_stage<2, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
     Ns *= R;
   }
 }
@@ -322,7 +334,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
     #pragma unroll
     for(int u = 0; u < FFT_G; u++){
       const int e = e0 + u * blockDim.x;
-      const int t = e % ti, j0 = e / ti, i = i0 + t;
+      const int t = (e & (ti - 1)), j0 = (e >> fft_lg(ti)), i = i0 + t;
       const bool ok = e < nel && i <= g.nx;
       const long long o = tck_idx(g, ok ? i : 1, ok ? j0 + 1 : 1, k);
       P[u][0] = A.ux[o]; P[u][1] = A.ux[o + 1]; P[u][2] = A.uy[o]; P[u][3] = A.uy[o + sy];
@@ -332,7 +344,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
     for(int u = 0; u < FFT_G; u++){
       const int e = e0 + u * blockDim.x;
       if(e >= nel) break;
-      const int t = e % ti, j0 = e / ti, i = i0 + t;
+      const int t = (e & (ti - 1)), j0 = (e >> fft_lg(ti)), i = i0 + t;
       double val = 0.;
       if(i <= g.nx){
         const double div = c.ijdxc[i] * (
@@ -349,8 +361,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
   /* split: X_k = (Z_k + conj Z_{M-k}) / 2 - i W^k (Z_k - conj Z_{M-k}) / 2, k = 0..M */
   V * S = (V *)A.S;
   for(int e = threadIdx.x; e < ti * (M + 1); e += blockDim.x){
-    const int t = e % ti;
-    const int kk = e / ti;
+    const int t = (e & (ti - 1));
+    const int kk = (e >> fft_lg(ti));
     const int i = i0 + t;
     if(i > g.nx) continue;
     const V zk = sm[t * A.ld + fpad(kk == M ? 0 : kk)];
@@ -398,7 +410,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
   V * S = (V *)A.S;
   /* element q of this thread in a tile: line t0, row j = r0 + q dr (the block size is a
    * multiple of the tile width, a power of two no larger than 64) */
-  const int t0i = threadIdx.x % ti, r0i = threadIdx.x / ti, dr = nt / ti;
+  const int t0i = (threadIdx.x & (ti - 1)), r0i = (threadIdx.x >> fft_lg(ti)), dr = (nt >> fft_lg(ti));
   int t0 = t0i, r0 = r0i;
   double pre[PF][6];
   bool pok[PF];
@@ -482,8 +494,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
     }
     lds_fft<T, EPTK, W>(sm, A.ld, ti, A.P, twM, T(-1.));
     for(int e = threadIdx.x; e < ti * (M + 1); e += nt){
-      const int t = e % ti;
-      const int kk = e / ti;
+      const int t = (e & (ti - 1));
+      const int kk = (e >> fft_lg(ti));
       const int ii = i0 + t;
       if(ii > g.nx) continue;
       const V zk = sm[t * A.ld + fpad(kk == M ? 0 : kk)];
@@ -533,22 +545,22 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
     #pragma unroll
     for(int u = 0; u < FFT_GZ; u++){
       const int e = e0 + u * blockDim.x;
-      const int t = e % tz, kz = e / tz, ii = i0 + t;
+      const int t = (e & (tz - 1)), kz = (e >> fft_lg(tz)), ii = i0 + t;
       v[u] = tck_ld((e < nel && ii < A.nx) ? &base[ii + kstride * kz] : (const V *)A.S);
     }
     #pragma unroll
     for(int u = 0; u < FFT_GZ; u++){
       const int e = e0 + u * blockDim.x;
       if(e >= nel) break;
-      const int t = e % tz, kz = e / tz;
+      const int t = (e & (tz - 1)), kz = (e >> fft_lg(tz));
       sm[t * A.ld + fpad(kz)] = i0 + t < A.nx ? v[u] : mk2<T>(T(0.), T(0.));
     }
   }
   __syncthreads();
   lds_fft<T, FFT_EPT, W>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
   for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
-    const int t = e % tz;
-    const int kz = e / tz;
+    const int t = (e & (tz - 1));
+    const int kz = (e >> fft_lg(tz));
     const int ii = i0 + t;
     if(ii < A.nx) tck_st(&base[ii + kstride * kz], sm[t * A.ld + fpad(kz)]);
   }
@@ -575,7 +587,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
   const long long kstride = (long long)A.nx * A.nky;
   /* element q of this thread in a tile: line t0, row kz = r0 + q dr (the block size is a
    * multiple of the tile width, a power of two no larger than 64) */
-  const int t0i = threadIdx.x % tz, r0i = threadIdx.x / tz, dr = nt / tz;
+  const int t0i = (threadIdx.x & (tz - 1)), r0i = (threadIdx.x >> fft_lg(tz)), dr = (nt >> fft_lg(tz));
   int t0 = t0i, r0 = r0i;
   V pre[PF];
   bool pok[PF];
@@ -676,7 +688,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     #pragma unroll
     for(int u = 0; u < FFT_GZ; u++){
       const int e = e0 + u * blockDim.x;
-      const int t = e % ti, kk = e / ti, i = i0 + t;
+      const int t = (e & (ti - 1)), kk = (e >> fft_lg(ti)), i = i0 + t;
       const bool ok = e < nel && i <= g.nx;
       const long long o = (long long)(i - 1) + rowbase;
       v[u] = tck_ld(ok ? &S[o + (long long)g.nx * kk] : S);
@@ -686,7 +698,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     for(int u = 0; u < FFT_GZ; u++){
       const int e = e0 + u * blockDim.x;
       if(e >= nel) break;
-      const int t = e % ti, kk = e / ti, i = i0 + t;
+      const int t = (e & (ti - 1)), kk = (e >> fft_lg(ti)), i = i0 + t;
       V w = mk2<T>(T(0.), T(0.));
       if(i <= g.nx){
         w = v[u];
@@ -704,8 +716,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
   /* pairs (k, M-k), k = 1..M/2, in place */
   const int half = M / 2;
   for(int e = threadIdx.x; e < ti * half; e += blockDim.x){
-    const int t = e % ti;
-    const int k1 = 1 + e / ti;
+    const int t = (e & (ti - 1));
+    const int k1 = 1 + (e >> fft_lg(ti));
     const int k2 = M - k1;
     const V xa = sm[t * A.ld + fpad(k1)];
     const V xb = sm[t * A.ld + fpad(k2)];
@@ -720,8 +732,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
   lds_fft<T, FFT_EPT, W>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
   T * psi = (T *)A.psi;
   for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
-    const int t = e % ti;
-    const int jj = e / ti;
+    const int t = (e & (ti - 1));
+    const int jj = (e >> fft_lg(ti));
     const int i = i0 + t;
     if(i > g.nx) continue;
     const T v = ((const T *)(sm + t * A.ld))[2 * fpad(jj >> 1) + (jj & 1)];
@@ -753,7 +765,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
   /* element q of this thread in a tile: line t0, wavenumber kk = r0 + q dr (the block size is
    * a multiple of the tile width, a power of two no larger than 64); only q = 0 can hold
    * kk = 0 */
-  const int t0i = threadIdx.x % ti, r0i = threadIdx.x / ti, dr = nt / ti;
+  const int t0i = (threadIdx.x & (ti - 1)), r0i = (threadIdx.x >> fft_lg(ti)), dr = (nt >> fft_lg(ti));
   int t0 = t0i, r0 = r0i;
   V pre[PF];
   bool pok[PF];
@@ -814,8 +826,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
     }
     const int half = M / 2;
     for(int e = threadIdx.x; e < ti * half; e += nt){
-      const int t = e % ti;
-      const int k1 = 1 + e / ti;
+      const int t = (e & (ti - 1));
+      const int k1 = 1 + (e >> fft_lg(ti));
       const int k2 = M - k1;
       const V xa = sm[t * A.ld + fpad(k1)];
       const V xb = sm[t * A.ld + fpad(k2)];
@@ -829,8 +841,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
     fft_lds_barrier();
     lds_fft<T, EPTK, W>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
     for(int e = threadIdx.x; e < ti * g.ny; e += nt){
-      const int t = e % ti;
-      const int jj = e / ti;
+      const int t = (e & (ti - 1));
+      const int jj = (e >> fft_lg(ti));
       const int i = i0 + t;
       if(i > g.nx) continue;
       const T v = ((const T *)(sm + t * A.ld))[2 * fpad(jj >> 1) + (jj & 1)];
@@ -861,6 +873,14 @@ static int make_plan(int M, FftPlan * P){
     if(P->nst >= FFT_MAXST) return -1;
     P->rad[P->nst++] = r;
     n /= r;
+  }
+  /* division multipliers of each stage (dividends stay below 16 * 4096 = 2^16) */
+  int ns = 1;
+  for(int st = 0; st < P->nst; st++){
+    const unsigned per = (unsigned)(M / P->rad[st]);
+    P->mper[st] = per > 1 ? (unsigned)(0x100000000ULL / per + 1) : 0u;
+    P->mns[st] = ns > 1 ? (unsigned)(0x100000000ULL / (unsigned)ns + 1) : 0u;
+    ns *= P->rad[st];
   }
   return 0;
 }
@@ -929,6 +949,8 @@ static int fft_ld(int M){
   return ld | 1;
 }
 
+/* the tile width b is a power of two (16 halved or doubled, capped at a power of two): the
+ * kernels divide by it with shifts */
 static int fft_tile(int M, int cap, size_t csize, int * threads, int share, long long lines){
   int b = 16;
   const size_t lb = (size_t)fft_ld(M) * csize;
