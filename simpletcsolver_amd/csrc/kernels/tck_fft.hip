@@ -612,8 +612,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
     t0 = t0i; r0 = r0i;
     asm volatile("" : "+v"(t0), "+v"(r0));
     /* the rest of the tile: all loads issued before the first is waited for */
-    V cur[EPTK - PF];
-    bool cok[EPTK - PF];
+    V cur[EPTK - PF > 0 ? EPTK - PF : 1];
+    bool cok[EPTK - PF > 0 ? EPTK - PF : 1];
     #pragma unroll
     for(int q = PF; q < EPTK; q++){ ZF_ELEM(tile, q, cur[q - PF], cok[q - PF]); }
     #pragma unroll
@@ -810,8 +810,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
       sm[t0 * A.ld + fpad(kk)] = v;
     };
     /* the rest of the tile: all loads issued before the first is waited for */
-    V cur[EPTK - PF];
-    bool cok[EPTK - PF];
+    V cur[EPTK - PF > 0 ? EPTK - PF : 1];
+    bool cok[EPTK - PF > 0 ? EPTK - PF : 1];
     #pragma unroll
     for(int q = PF; q < EPTK; q++){ YI_ELEM(tile, q, cur[q - PF], cok[q - PF]); }
     #pragma unroll
