@@ -62,7 +62,7 @@ struct tc_solver_s {
   double * d_psi;
   double * d_tab;
   void * d_spec;
-  double * d_pr; double * d_pv; double * d_pl; double * d_pu;
+  double * d_pr; double * d_pl; double * d_pu;
   double * d_evy; double * d_evz;
   void * d_twy; void * d_twyn; void * d_twz;   /* double2, or float2 in the mixed step */
   double * d_dt;

@@ -48,9 +48,9 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, int nf, void * const * d, do
     const int * i0, const int * isz, int mixed, hipStream_t stream);
 int tck_launch_pfactor(int nx, int nky, int nz, const double * pl, const double * pu,
     const double * hyxc, const double * evy, const double * evz, double hz, double * pr,
-    double * pv, int exact_gauge, hipStream_t stream);
-int tck_launch_pxsolve(int nx, int nlines, void * s, const double * pr, const double * pv,
-    const double * pl, int mixed, hipStream_t stream);
+    int exact_gauge, hipStream_t stream);
+int tck_launch_pxsolve(int nx, int nlines, void * s, const double * pr, const double * pl,
+    const double * pu, int mixed, hipStream_t stream);
 
 /* tck_fft.hip */
 int tck_fft_supported(int n);

@@ -305,9 +305,9 @@ static int setup_poisson(tc_solver_t * s){
   free(evz);
   const size_t nspec = (size_t)nx * nky * nz;
   if(dalloc_bytes(s, &s->d_spec, 2 * s->esz * nspec)) return 1;
-  if(dalloc(s, &s->d_pr, nspec) || dalloc(s, &s->d_pv, nspec)) return 1;
+  if(dalloc(s, &s->d_pr, nspec)) return 1;
   KOK(tck_launch_pfactor(nx, nky, nz, s->d_pl, s->d_pu, s->coef.hyxc, s->d_evy, s->d_evz,
-      s->hz, s->d_pr, s->d_pv, s->mixed, s->stream));
+      s->hz, s->d_pr, s->mixed, s->stream));
   if(upload_twiddles(s, ny / 2, ny / 2, &s->d_twy)) return 1;
   if(upload_twiddles(s, ny, ny / 2 + 1, &s->d_twyn)) return 1;
   if(upload_twiddles(s, nz, nz, &s->d_twz)) return 1;
@@ -865,7 +865,7 @@ static int potential(tc_solver_t * s, int rkstep){
   PROF(TC_S_DIVY, tck_launch_div_yfft(g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
       tc_field_ptr(s, TC_FUZ), s->d_dt, rkstep, s->d_spec, s->d_twy, s->d_twyn, mx, s->stream));
   PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, -1, mx, s->stream));
-  PROF(TC_S_PXSOLVE, tck_launch_pxsolve(g->nx, s->nky * g->nz, s->d_spec, s->d_pr, s->d_pv, s->d_pl, mx, s->stream));
+  PROF(TC_S_PXSOLVE, tck_launch_pxsolve(g->nx, s->nky * g->nz, s->d_spec, s->d_pr, s->d_pl, s->d_pu, mx, s->stream));
   PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, +1, mx, s->stream));
   PROF(TC_S_YIFFT, tck_launch_yifft_psi(g, s->d_spec, psi, s->d_twy, s->d_twyn, mx, s->stream));
   return 0;

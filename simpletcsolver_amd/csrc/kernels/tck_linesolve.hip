@@ -774,7 +774,9 @@ __global__ void __launch_bounds__(512) tck_psolve(const PSolveArgs A){
 /* ---------------------------------------------------------------------------------------
  * Poisson radial systems (compute_potential.c:445-488): per line (ky, kz)
  *   c_i = - l_i - u_i + ly(ky) / HYXC(i)^2 + lz(kz) / hz^2,  c_0 += l_0,  c_{n-1} += u_{n-1}
- * factors precomputed once into pr/pv (layout [line][i]); rows of the spectrum are complex.
+ * factors r precomputed once into pr (layout [line][i]); the other factor, v_i = r_i u_i
+ * (v_0 = u_0 / c_0, v_{n-1} = 0), is formed again by the solve with the same operation, so
+ * that only r is read; rows of the spectrum are complex.
  * ------------------------------------------------------------------------------------- */
 struct PFactorArgs {
   int nx, nky, nz;
@@ -783,7 +785,7 @@ struct PFactorArgs {
   const double * hyxc;                      /* HYXC(i), reference index */
   const double * evy; const double * evz;
   double hz;
-  double * pr; double * pv;
+  double * pr;
 };
 
 __global__ void tck_pfactor(const PFactorArgs A){
@@ -795,20 +797,17 @@ __global__ void tck_pfactor(const PFactorArgs A){
   const double eval_z = A.evz[kz];
   const int n = A.nx;
   double * r = A.pr + line * n;
-  double * v = A.pv + line * n;
   const double hz = A.hz;
   /* the Neumann fix-ups are added after the sum, as c[0] += l[0] does (:481-482) */
   double c0 = - A.pl[0] - A.pu[0] + eval_y / A.hyxc[1] / A.hyxc[1] + eval_z / hz / hz;
   c0 += A.pl[0];
   double vprev = A.pu[0] / c0;
   r[0] = c0;
-  v[0] = vprev;
   for(int i = 1; i < n - 1; i++){
     const double ci = - A.pl[i] - A.pu[i] + eval_y / A.hyxc[i + 1] / A.hyxc[i + 1] + eval_z / hz / hz;
     const double val = 1. / (ci - A.pl[i] * vprev);
     vprev = val * (A.pu[i]);
     r[i] = val;
-    v[i] = vprev;
   }
   double cl = - A.pl[n - 1] - A.pu[n - 1] + eval_y / A.hyxc[n] / A.hyxc[n] + eval_z / hz / hz;
   cl += A.pu[n - 1];
@@ -819,40 +818,39 @@ __global__ void tck_pfactor(const PFactorArgs A){
    * side sums to zero to FP64 rounding; an FP32 spectrum does not, and the division would
    * swamp psi with a huge constant.  The mixed step fixes the gauge exactly instead. */
   if(A.exact_gauge && line == 0) r[n - 1] = 0.;
-  v[n - 1] = 0.;
 }
 
 struct PXSolveArgs {
   int nx, nlines;
   void * s;                  /* spectrum, line-major rows of nx complex (double2, or float2
                               * in the mixed step; the solve itself runs in FP64) */
-  const double * pr; const double * pv; const double * pl;
+  const double * pr; const double * pl; const double * pu;
 };
 
 /* one (ky, kz) line per wave per iteration, WPB waves per block looping over groups of WPB
- * lines; the spectrum row and its two factor rows are staged through LDS so that every
- * global access is contiguous across the wave (WPB keeps the staging area within 64 KB) */
+ * lines; the spectrum row and its factor row are staged through LDS so that every global
+ * access is contiguous across the wave (WPB keeps the staging area within 64 KB) */
 template <int SEG, int WPB, typename C2>
 __global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs A){
   __shared__ double2 smq_all[WPB][TCK_WAVE * (SEG + 1)];
   __shared__ double smr_all[WPB][TCK_WAVE * (SEG + 1)];
-  __shared__ double smv_all[WPB][TCK_WAVE * (SEG + 1)];
   const int w = threadIdx.x / TCK_WAVE;
   const int lane = threadIdx.x % TCK_WAVE;
   double2 * smq = smq_all[w];
   double * smr = smr_all[w];
-  double * smv = smv_all[w];
   const int n = A.nx;
   const int row0 = lane * SEG;
-  double ll[SEG];
+  double ll[SEG], uu[SEG];
   #pragma unroll
-  for(int e = 0; e < SEG; e++) ll[e] = A.pl[row0 + e < n ? row0 + e : n - 1];
+  for(int e = 0; e < SEG; e++){
+    ll[e] = A.pl[row0 + e < n ? row0 + e : n - 1];
+    uu[e] = A.pu[row0 + e < n ? row0 + e : n - 1];
+  }
   for(long long g0 = (long long)blockIdx.x * WPB; g0 < A.nlines; g0 += (long long)gridDim.x * WPB){
     const long long line = g0 + w;
     const bool live = line < A.nlines;
     C2 * row = (C2 *)A.s + (live ? line : 0) * n;
     const double * r = A.pr + (live ? line : 0) * n;
-    const double * v = A.pv + (live ? line : 0) * n;
     if(live){
       #pragma unroll
       for(int e = 0; e < SEG; e++){
@@ -862,7 +860,6 @@ __global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs 
           const C2 x = row[idx];
           smq[p] = make_double2(double(x.x), double(x.y));
           smr[p] = r[idx];
-          smv[p] = v[idx];
         }
       }
     }
@@ -874,7 +871,9 @@ __global__ void __launch_bounds__(WPB * TCK_WAVE) tck_pxsolve(const PXSolveArgs 
       q[e][0] = x.x;
       q[e][1] = x.y;
       rr[e] = smr[lane * (SEG + 1) + e];
-      vv[e] = smv[lane * (SEG + 1) + e];
+      /* v as the factor kernel forms it: u_0 / c_0 (r_0 holds c_0), r_i u_i, 0 at the end */
+      const int row = row0 + e;
+      vv[e] = row == 0 ? uu[e] / rr[e] : (row == n - 1 ? 0. : rr[e] * uu[e]);
     }
     wave_tdm<SEG, 2>(q, rr, ll, vv, n);
     #pragma unroll
@@ -1020,21 +1019,21 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, int nf, void * const * d, do
 
 int tck_launch_pfactor(int nx, int nky, int nz, const double * pl, const double * pu,
     const double * hyxc, const double * evy, const double * evz, double hz, double * pr,
-    double * pv, int exact_gauge, hipStream_t stream){
+    int exact_gauge, hipStream_t stream){
   PFactorArgs A;
   A.nx = nx; A.nky = nky; A.nz = nz; A.exact_gauge = exact_gauge; A.pl = pl; A.pu = pu; A.hyxc = hyxc; A.evy = evy;
-  A.evz = evz; A.hz = hz; A.pr = pr; A.pv = pv;
+  A.evz = evz; A.hz = hz; A.pr = pr;
   const long long nlines = (long long)nky * nz;
   hipLaunchKernelGGL(tck_pfactor, dim3((unsigned)((nlines + 63) / 64)), dim3(64), 0, stream, A);
   TCK_CHECK_LAUNCH();
   return 0;
 }
 
-int tck_launch_pxsolve(int nx, int nlines, void * s, const double * pr, const double * pv,
-    const double * pl, int mixed, hipStream_t stream){
+int tck_launch_pxsolve(int nx, int nlines, void * s, const double * pr, const double * pl,
+    const double * pu, int mixed, hipStream_t stream){
   if(nx > TCK_WAVE * XS_MAXSEG || nx < 2) return (int)hipErrorInvalidValue;
   PXSolveArgs A;
-  A.nx = nx; A.nlines = nlines; A.s = s; A.pr = pr; A.pv = pv; A.pl = pl;
+  A.nx = nx; A.nlines = nlines; A.s = s; A.pr = pr; A.pl = pl; A.pu = pu;
   switch(seg_for(nx)){
     case 1: return launch_pxsolve_t<1>(A, mixed, stream);
     case 2: return launch_pxsolve_t<2>(A, mixed, stream);
