@@ -121,6 +121,14 @@ template <> __device__ __forceinline__ float tck_bld<float>(__amdgpu_buffer_rsrc
 __device__ __forceinline__ void tck_bst(__amdgpu_buffer_rsrc_t r, unsigned vo, double v){
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, 0, TCK_NT_STORE ? 2 : 0);
 }
+/* 16-byte store of zeros at byte offset vo of resource r; with a zero-size resource it writes
+ * nothing but is counted among the outstanding memory operations like any store (distinct
+ * offsets keep the compiler from merging such stores) */
+__device__ __forceinline__ void tck_bst0(__amdgpu_buffer_rsrc_t r, unsigned vo){
+  typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+  const u4v z = {0u, 0u, 0u, 0u};
+  __builtin_amdgcn_raw_buffer_store_b128(z, r, vo, 0u, 0);
+}
 /* store of a halo image (plain, as tck_store_halo's) */
 __device__ __forceinline__ void tck_bsti(__amdgpu_buffer_rsrc_t r, unsigned vo, double v){
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, 0, 0);

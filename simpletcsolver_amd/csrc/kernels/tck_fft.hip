@@ -48,6 +48,11 @@ struct FftPlan {
   /* per stage: multipliers that divide by M / rad and by the stage's span Ns with one high
    * multiply (floor(2^32 / d) + 1, exact for dividends below 2^16; 0 for d = 1) */
   unsigned mper[FFT_MAXST], mns[FFT_MAXST];
+  /* threads per block of the launch.  The kernels read the block size from here (a kernel
+   * argument, in scalar registers) rather than from blockDim.x, whose 16-bit read from the
+   * dispatch packet is a vector memory load that the compiler may repeat inside the passes:
+   * a wait for it there would also wait for every global load and store issued before it. */
+  int nt;
 };
 
 /* log2 of a tile width (tile widths are powers of two, at most 64) */
@@ -209,10 +214,9 @@ __host__ __device__ __forceinline__ int fpad(int n){ return n + (n >> 3); }
 
 template <int R, typename T, int EPT = FFT_EPT>
 __device__ __forceinline__ void fft_stage(typename cvec<T>::type * buf, int M, int ld, int nb, int Ns,
-    const typename cvec<T>::type * __restrict__ tw, T s, unsigned mper, unsigned mns){
+    const typename cvec<T>::type * __restrict__ tw, T s, unsigned mper, unsigned mns, int nt){
   typedef typename cvec<T>::type V;
   constexpr int BPT = (EPT + R - 1) / R;
-  const int nt = blockDim.x;
   const int tid = threadIdx.x;
   const int per = M / R;
   const int nbf = nb * per;
@@ -267,12 +271,12 @@ __device__ void lds_fft(typename cvec<T>::type * buf, int ld, int nb, const FftP
   for(int st = 0; st < P.nst; st++){
     const int R = P.rad[st];
     const unsigned mp = P.mper[st], mn = P.mns[st];
-    if(R == 8) fft_stage<8, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
-    else if(R == 4) fft_stage<4, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
-    else if(R == 3) fft_stage<3, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
-    else if(W && R == 5) fft_stage<5, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
-    else if(W && R == 7) fft_stage<7, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
-    else if(R == 2) fft_stage<2, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn);
+    if(R == 8) fft_stage<8, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn, P.nt);
+    else if(R == 4) fft_stage<4, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn, P.nt);
+    else if(R == 3) fft_stage<3, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn, P.nt);
+    else if(W && R == 5) fft_stage<5, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn, P.nt);
+    else if(W && R == 7) fft_stage<7, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn, P.nt);
+    else if(R == 2) fft_stage<2, T, EPT>(buf, P.M, ld, nb, Ns, tw, s, mp, mn, P.nt);
     Ns *= R;
   }
 }
@@ -329,11 +333,11 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
    * together (out-of-field elements read cell (1, 1) of the plane and are put as 0, so that no
    * load is conditional) */
   const int nel = ti * g.ny;
-  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_G * blockDim.x){
+  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_G * A.P.nt){
     double P[FFT_G][6];
     #pragma unroll
     for(int u = 0; u < FFT_G; u++){
-      const int e = e0 + u * blockDim.x;
+      const int e = e0 + u * A.P.nt;
       const int t = (e & (ti - 1)), j0 = (e >> fft_lg(ti)), i = i0 + t;
       const bool ok = e < nel && i <= g.nx;
       const long long o = tck_idx(g, ok ? i : 1, ok ? j0 + 1 : 1, k);
@@ -342,7 +346,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
     }
     #pragma unroll
     for(int u = 0; u < FFT_G; u++){
-      const int e = e0 + u * blockDim.x;
+      const int e = e0 + u * A.P.nt;
       if(e >= nel) break;
       const int t = (e & (ti - 1)), j0 = (e >> fft_lg(ti)), i = i0 + t;
       double val = 0.;
@@ -360,7 +364,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
   lds_fft<T, FFT_EPT, W>(sm, A.ld, ti, A.P, twM, T(-1.));
   /* split: X_k = (Z_k + conj Z_{M-k}) / 2 - i W^k (Z_k - conj Z_{M-k}) / 2, k = 0..M */
   V * S = (V *)A.S;
-  for(int e = threadIdx.x; e < ti * (M + 1); e += blockDim.x){
+  for(int e = threadIdx.x; e < ti * (M + 1); e += A.P.nt){
     const int t = (e & (ti - 1));
     const int kk = (e >> fft_lg(ti));
     const int i = i0 + t;
@@ -396,7 +400,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
   const tc_coef_t & c = A.c;
   const int ti = A.ti;
   const int M = A.P.M;
-  const int nt = blockDim.x;
+  const int nt = A.P.nt;
   const int ntx = (g.nx + ti - 1) / ti;
   const int ntiles = ntx * g.nz;
   const double dt = *A.dt;
@@ -540,17 +544,17 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   V * base = (V *)A.S + (long long)A.nx * ky;
   /* FFT_GZ elements loaded together; out-of-range ones read element 0 and are put as 0 */
   const int nel = tz * nz;
-  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_GZ * blockDim.x){
+  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_GZ * A.P.nt){
     V v[FFT_GZ];
     #pragma unroll
     for(int u = 0; u < FFT_GZ; u++){
-      const int e = e0 + u * blockDim.x;
+      const int e = e0 + u * A.P.nt;
       const int t = (e & (tz - 1)), kz = (e >> fft_lg(tz)), ii = i0 + t;
       v[u] = tck_ld((e < nel && ii < A.nx) ? &base[ii + kstride * kz] : (const V *)A.S);
     }
     #pragma unroll
     for(int u = 0; u < FFT_GZ; u++){
-      const int e = e0 + u * blockDim.x;
+      const int e = e0 + u * A.P.nt;
       if(e >= nel) break;
       const int t = (e & (tz - 1)), kz = (e >> fft_lg(tz));
       sm[t * A.ld + fpad(kz)] = i0 + t < A.nx ? v[u] : mk2<T>(T(0.), T(0.));
@@ -558,7 +562,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
   }
   __syncthreads();
   lds_fft<T, FFT_EPT, W>(sm, A.ld, tz, A.P, (const V *)A.tw, T(A.sign));
-  for(int e = threadIdx.x; e < tz * nz; e += blockDim.x){
+  for(int e = threadIdx.x; e < tz * nz; e += A.P.nt){
     const int t = (e & (tz - 1));
     const int kz = (e >> fft_lg(tz));
     const int ii = i0 + t;
@@ -576,18 +580,62 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft(const ZArgs A){
 #define TCK_ZFFT_PF 2            /* elements per thread loaded one tile ahead */
 #endif
 
-template <typename T, int PF, int EPTK, bool W>
+template <typename T, int PF, int EPTK, bool W, bool X>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
   V * sm = (V *)fft_smem;
-  const int tz = A.tz, nz = A.nz, nt = blockDim.x;
+  const int tz = A.tz, nz = A.nz, nt = A.P.nt;
   const int ntx = (A.nx + tz - 1) / tz;
   const int ntiles = ntx * A.nky;
   const long long kstride = (long long)A.nx * A.nky;
   /* element q of this thread in a tile: line t0, row kz = r0 + q dr (the block size is a
    * multiple of the tile width, a power of two no larger than 64) */
   const int t0i = (threadIdx.x & (tz - 1)), r0i = (threadIdx.x >> fft_lg(tz)), dr = (nt >> fft_lg(tz));
+  if constexpr (X){
+    /* Exact form: the tiles cover the spectrum exactly (nx a multiple of tz, blockDim.x EPTK =
+     * tz nz), so no access is conditional; the twiddle factors are read from LDS (after the
+     * tile's lines), so that the passes issue no global loads; and every element of the next
+     * tile is loaded before the current tile's passes and stores.  Outstanding global loads and
+     * stores complete in issue order as far as the wait counter is concerned, so a wait for a
+     * load issued after a store also waits for the store: here the loads of tile n+1 are
+     * issued before the stores of tile n, and the put of tile n+1 waits only for its loads.
+     * The stores into a zero-size buffer before the loop (they write nothing) give the loop's
+     * entry the same outstanding loads-then-stores sequence as the end of an iteration, so that
+     * the compiler's wait at the put is the same counted wait on both paths into it. */
+    V * twl = sm + tz * A.ld;
+    for(int t = threadIdx.x; t < A.P.M; t += nt) twl[t] = ((const V *)A.tw)[t];
+    int tile = blockIdx.x;             /* the launch has at least four tiles per block */
+    V pre[EPTK];
+    #define ZX_OFF(tl, q) ((long long)A.nx * ((tl) / ntx) + (long long)(((tl) % ntx) * tz + t0i) + kstride * (long long)(r0i + (q) * dr))
+    #pragma unroll
+    for(int q = 0; q < EPTK; q++) pre[q] = tck_ld((const V *)A.S + ZX_OFF(tile, q));
+    {
+      const __amdgpu_buffer_rsrc_t nul = tck_rsrc(A.S, 0u);
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++) tck_bst0(nul, 16u * q);
+    }
+    for(;;){
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++) sm[t0i * A.ld + fpad(r0i + q * dr)] = pre[q];
+      fft_lds_barrier();
+      const int nxt = tile + (int)gridDim.x < ntiles ? tile + (int)gridDim.x : tile;
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++) pre[q] = tck_ld((const V *)A.S + ZX_OFF(nxt, q));
+      lds_fft<T, EPTK, W>(sm, A.ld, tz, A.P, twl, T(A.sign));
+      V * base = (V *)A.S + (long long)A.nx * (tile / ntx) + (long long)((tile % ntx) * tz + t0i);
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++){
+        const int kz = r0i + q * dr;
+        tck_st(&base[kstride * kz], sm[t0i * A.ld + fpad(kz)]);
+      }
+      fft_lds_barrier();
+      tile += (int)gridDim.x;
+      if(tile >= ntiles) break;
+    }
+    #undef ZX_OFF
+    return;
+  }
   int t0 = t0i, r0 = r0i;
   V pre[PF];
   bool pok[PF];
@@ -683,11 +731,11 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
    * loaded together (with X_M for a kk = 0 element); out-of-field elements read element 0 and
    * are put as 0 */
   const int nel = ti * M;
-  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_GZ * blockDim.x){
+  for(int e0 = threadIdx.x; e0 < nel; e0 += FFT_GZ * A.P.nt){
     V v[FFT_GZ], xm[FFT_GZ];
     #pragma unroll
     for(int u = 0; u < FFT_GZ; u++){
-      const int e = e0 + u * blockDim.x;
+      const int e = e0 + u * A.P.nt;
       const int t = (e & (ti - 1)), kk = (e >> fft_lg(ti)), i = i0 + t;
       const bool ok = e < nel && i <= g.nx;
       const long long o = (long long)(i - 1) + rowbase;
@@ -696,7 +744,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
     }
     #pragma unroll
     for(int u = 0; u < FFT_GZ; u++){
-      const int e = e0 + u * blockDim.x;
+      const int e = e0 + u * A.P.nt;
       if(e >= nel) break;
       const int t = (e & (ti - 1)), kk = (e >> fft_lg(ti)), i = i0 + t;
       V w = mk2<T>(T(0.), T(0.));
@@ -715,7 +763,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
   __syncthreads();
   /* pairs (k, M-k), k = 1..M/2, in place */
   const int half = M / 2;
-  for(int e = threadIdx.x; e < ti * half; e += blockDim.x){
+  for(int e = threadIdx.x; e < ti * half; e += A.P.nt){
     const int t = (e & (ti - 1));
     const int k1 = 1 + (e >> fft_lg(ti));
     const int k2 = M - k1;
@@ -731,7 +779,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
   __syncthreads();
   lds_fft<T, FFT_EPT, W>(sm, A.ld, ti, A.P, (const V *)A.twM, T(1.));
   T * psi = (T *)A.psi;
-  for(int e = threadIdx.x; e < ti * g.ny; e += blockDim.x){
+  for(int e = threadIdx.x; e < ti * g.ny; e += A.P.nt){
     const int t = (e & (ti - 1));
     const int jj = (e >> fft_lg(ti));
     const int i = i0 + t;
@@ -756,7 +804,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
   const tc_geom_t & g = A.g;
   const int ti = A.ti;
   const int M = A.P.M;
-  const int nt = blockDim.x;
+  const int nt = A.P.nt;
   const int ntx = (g.nx + ti - 1) / ti;
   const int ntiles = ntx * g.nz;
   const V * S = (const V *)A.S;
@@ -1060,6 +1108,7 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
   A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE, (long long)g->nx * g->nz);
+  A.P.nt = nt;
   A.twM = twM; A.twN = twN;
   A.ld = fft_ld(A.P.M);
   const size_t lds = cs * (size_t)A.ti * A.ld;
@@ -1079,7 +1128,7 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
   return 0;
 }
 
-int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign, int mixed, hipStream_t stream){
+int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign, int mixed, int exact, hipStream_t stream){
   ZArgs A;
   A.nx = nx; A.nky = nky; A.nz = nz; A.S = S; A.tw = tw;
   A.sign = sign < 0 ? -1. : 1.;
@@ -1087,14 +1136,21 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
   A.tz = fft_tile(nz, 64, cs, &nt, TCK_FFT_ZSHARE, (long long)nx * nky);
+  A.P.nt = nt;
   A.ld = fft_ld(nz);
   const size_t lds = cs * (size_t)A.tz * A.ld;
   const long long ntiles = (long long)((nx + A.tz - 1) / A.tz) * nky;
 #if TCK_ZFFT_PF > 0
   {
     const long long el = (long long)A.tz * nz;
-    const int r = mixed ? 0 : (plan_wide(&A.P) ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, true>, A, nt, lds, el, ntiles, stream)
-                                   : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false>, A, nt, lds, el, ntiles, stream));
+    /* the exact form when the tiles cover the spectrum exactly and its twiddle factors fit
+     * in the LDS beside the tile */
+    const size_t ldsx = lds + cs * (size_t)nz;
+    const bool x = exact && nx % A.tz == 0 && (long long)nt * PERSIST_EPT == el && ldsx <= lds_limit();
+    const int r = mixed ? 0 : x ? (plan_wide(&A.P) ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, true, true>, A, nt, ldsx, el, ntiles, stream)
+                                       : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream))
+                    : (plan_wide(&A.P) ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, true, false>, A, nt, lds, el, ntiles, stream)
+                                       : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
@@ -1113,6 +1169,7 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
   const size_t cs = mixed ? sizeof(float2) : sizeof(double2);
   int nt;
   A.ti = fft_tile(A.P.M, 64, cs, &nt, TCK_FFT_YSHARE, (long long)g->nx * g->nz);
+  A.P.nt = nt;
   A.twM = twM; A.twN = twN;
   A.ld = fft_ld(A.P.M);
   const size_t lds = cs * (size_t)A.ti * A.ld;

@@ -135,6 +135,31 @@ def test_device_manufactured_potential_long_lines(need_gpu):
     assert err <= 1e-9
 
 
+def test_device_exact_transform_form_computes_the_same_bits(need_gpu):
+    """at 128 x 256 x 1024 the persistent axial transforms take their exact-tiling form (tiles
+    of 8 lines cover nx = 128, 1024 threads x 8 elements cover nz = 1024, twiddle factors in
+    LDS, the next tile loaded in full before the current one's passes); the general persistent
+    form computes the same potential bit for bit"""
+    from simpletcsolver_amd import Solver
+    nx, ny, nz = 128, 256, 1024
+    small = make_case(nx=nx, ny=ny, nz=4, Re=100.0, amp=0.0, uy_walls=(0.0, 0.0))
+    rng = np.random.default_rng(41)
+    f = {"ux": rng.standard_normal((nz, ny, nx + 1)), "uy": rng.standard_normal((nz, ny, nx + 2)),
+         "uz": rng.standard_normal((nz, ny, nx + 2))}
+    f["ux"][:, :, 0] = 0.0
+    f["ux"][:, :, -1] = 0.0
+    out = []
+    for exact in (1, 0):
+        s = Solver(nx, ny, nz, small["xf"], small["xc"], **small["kw"])
+        s.tune("fft_exact", exact, 0)
+        s.load_fields(f)
+        s.potential(1, 0.13)
+        out.append(s.get_field("psi"))
+        s.close()
+    assert np.array_equal(out[0], out[1])
+    assert np.abs(out[0]).max() > 0.0
+
+
 @pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 1, 1)])
 def test_device_step_is_solenoidal(need_gpu, implicit):
     case = make_case(nx=24, ny=16, nz=12, Re=300.0, implicit=implicit, amp=0.3)
