@@ -90,7 +90,7 @@ struct tc_solver_s {
   tc_prof_t prof[TC_NSTAGE];
   int tune[TC_NTUNE][2];
   int predict_window;     /* 1: the predictor variant with the fields' planes staged in LDS */
-  int fft_exact;          /* 1: the exact-tiling form of the persistent transforms where it applies */
+  int fft_exact;          /* exact-tiling forms of the persistent transforms where they apply: 1 axial, 4 inverse theta */
   /* bumped by every host-side change that work recorded earlier (an ensemble's graphs)
    * cannot see: fields replaced, a closure coupled, uncoupled or destroyed, launch shapes,
    * dt parameters, diagnostics or profiling armed, and every launch outside a replay */

@@ -60,7 +60,7 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
     const void * twM, const void * twN, int mixed, hipStream_t stream);
 int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign, int mixed, int exact, hipStream_t stream);
 int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const void * twM,
-    const void * twN, int mixed, hipStream_t stream);
+    const void * twN, int mixed, int exact, hipStream_t stream);
 
 /* tck_util.hip: device address of this device's staging array of the allocation-free
  * initial-condition fill (4 tables of TC_IC_NOALLOC_N doubles), looked up once per device */

@@ -25,7 +25,8 @@ __device__ __forceinline__ long long tck_idx(const tc_geom_t & g, int i, int j, 
 /* Store an interior value (1 <= j <= ny, 1 <= k <= nz) together with its periodic halo
  * images (src/halo.c:18-142 fills j = 0 from ny, j = ny+1 from 1, then the z planes over the
  * full (x, y) extent, so corners are images of images).  With xcopy set the value is also
- * written to the x wall slot next to it (Neumann copy of src/fluid/boundary/{p,psi}.c). */
+ * written to the x wall slot next to it (Neumann copy of src/fluid/boundary/{p,psi}.c).
+ * MAIN false: the images and copies only, for a caller that has stored the value itself. */
 typedef double tck_d2v __attribute__((ext_vector_type(2)));
 typedef float tck_f2v __attribute__((ext_vector_type(2)));
 #ifndef TCK_NT_LOAD
@@ -83,7 +84,7 @@ __device__ __forceinline__ void tck_st(float2 * p, float2 v){
 #endif
 }
 
-template <typename T>
+template <typename T, bool MAIN = true>
 __device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i, int j, int k, T v, bool xcopy){
   const int nj = (j == 1) ? g.ny + 1 : (j == g.ny) ? 0 : -1;
   const int nk = (k == 1) ? g.nz + 1 : (k == g.nz) ? 0 : -1;
@@ -92,7 +93,7 @@ __device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i
   for(int a0 = 0; a0 < 2; a0++){
     const int ii = a0 ? ni : i;
     if(ii < 0) continue;
-    tck_st(&a[tck_idx(g, ii, j, k)], v);
+    if(MAIN || a0) tck_st(&a[tck_idx(g, ii, j, k)], v);
     if(nj >= 0) a[tck_idx(g, ii, nj, k)] = v;
     if(nk >= 0){
       a[tck_idx(g, ii, j, nk)] = v;
@@ -121,12 +122,21 @@ template <> __device__ __forceinline__ float tck_bld<float>(__amdgpu_buffer_rsrc
 __device__ __forceinline__ void tck_bst(__amdgpu_buffer_rsrc_t r, unsigned vo, double v){
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, 0, TCK_NT_STORE ? 2 : 0);
 }
+/* the same with a uniform byte offset so added to vo */
+__device__ __forceinline__ void tck_bsts(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, double v){
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, so, TCK_NT_STORE ? 2 : 0);
+}
+/* complex FP64 value: load of a streamed operand (cache policy as tck_ld) at byte offset
+ * vo + so of resource r */
+typedef unsigned int tck_u4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double2 tck_bld2(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
+  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, TCK_NT_LOAD ? 2 : 0));
+}
 /* 16-byte store of zeros at byte offset vo of resource r; with a zero-size resource it writes
  * nothing but is counted among the outstanding memory operations like any store (distinct
  * offsets keep the compiler from merging such stores) */
 __device__ __forceinline__ void tck_bst0(__amdgpu_buffer_rsrc_t r, unsigned vo){
-  typedef unsigned int u4v __attribute__((ext_vector_type(4)));
-  const u4v z = {0u, 0u, 0u, 0u};
+  const tck_u4v z = {0u, 0u, 0u, 0u};
   __builtin_amdgcn_raw_buffer_store_b128(z, r, vo, 0u, 0);
 }
 /* store of a halo image (plain, as tck_store_halo's) */

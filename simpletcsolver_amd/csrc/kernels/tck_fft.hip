@@ -508,8 +508,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
       const V ev = mk2<T>(T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y + zc.y));
       const V od = mk2<T>(T(0.5) * (zk.x - zc.x), T(0.5) * (zk.y - zc.y));
       const V wo = cmul(w, od);
-      const V X = mk2<T>(ev.x + wo.y, ev.y - wo.x);
-      tck_st(&S[(long long)(ii - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))], X);
+      const V Xk = mk2<T>(ev.x + wo.y, ev.y - wo.x);
+      tck_st(&S[(long long)(ii - 1) + (long long)g.nx * ((long long)kk + (long long)A.nky * (k - 1))], Xk);
     }
     fft_lds_barrier();
   }
@@ -796,7 +796,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi(const YInvArgs A
 #define TCK_YIFFT_PF 2
 #endif
 
-template <typename T, int PF, int EPTK, bool W>
+template <typename T, int PF, int EPTK, bool W, bool X>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -814,6 +814,110 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
    * a multiple of the tile width, a power of two no larger than 64); only q = 0 can hold
    * kk = 0 */
   const int t0i = (threadIdx.x & (ti - 1)), r0i = (threadIdx.x >> fft_lg(ti)), dr = (nt >> fft_lg(ti));
+  if constexpr (X){
+    /* Exact form: the tiles cover the field exactly (nx a multiple of ti, blockDim.x EPTK =
+     * ti M), so no load is conditional and the loops have fixed counts; the twiddle factors
+     * are read from LDS (after the tile's lines); every spectrum value of the next tile is
+     * loaded before the current tile's passes and stores, and the put waits only for those
+     * loads (the zero-size stores before the loop give its entry the outstanding
+     * loads-then-stores sequence of an iteration's end, as in tck_zfft_p). */
+    V * twl = sm + ti * A.ld;
+    V * twnl = twl + M;
+    const V * twM = (const V *)A.twM;
+    for(int t = threadIdx.x; t < M; t += nt){ twl[t] = twM[t]; twnl[t] = twN[t]; }
+    static_assert(sizeof(T) == 8, "the exact form is compiled for FP64 tiles");
+    constexpr int NST = 2 * EPTK;        /* psi values stored per thread: ti ny = 2 blockDim.x EPTK */
+    int tile = blockIdx.x;               /* the launch has at least four tiles per block */
+    /* buffer addressing: a tile's spectrum values lie in one plane of S (nky nx complex values)
+     * and its psi values in one plane of the field, so each access is a per-thread byte offset
+     * that does not change between tiles plus a uniform one */
+    const unsigned vs = 16u * (unsigned)(t0i + g.nx * r0i);            /* X_kk, kk = r0 + q dr */
+    const unsigned ss = 16u * (unsigned)(g.nx * dr);                   /* per q */
+    const unsigned vxm = 16u * (unsigned)(t0i + g.nx * M);             /* X_M */
+    const unsigned sbytes = 16u * (unsigned)(g.nx * A.nky);
+    const unsigned vp = 8u * (unsigned)(t0i + g.px * r0i);             /* psi row r0 + 1 + q dr */
+    const unsigned sp = 8u * (unsigned)(g.px * dr);
+    const unsigned pbytes = 8u * (unsigned)(g.px * g.ny);
+    #define YX_RS(tl) tck_rsrc(S + (long long)((tl) % ntx * ti) + (long long)g.nx * A.nky * ((tl) / ntx), sbytes)
+    V pre[EPTK], prexm;
+    {
+      const __amdgpu_buffer_rsrc_t rs = YX_RS(tile);
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++) pre[q] = tck_bld2(rs, vs, ss * q);
+      prexm = tck_bld2(rs, vxm, 0u);
+      const __amdgpu_buffer_rsrc_t nul = tck_rsrc(A.S, 0u);
+      #pragma unroll
+      for(int q = 0; q < NST; q++) tck_bst0(nul, 16u * q);
+    }
+    for(;;){
+      /* the LDS offsets are recomputed in every tile rather than held across the passes */
+      int t0 = t0i, r0 = r0i;
+      asm volatile("" : "+v"(t0), "+v"(r0));
+      const int i0 = 1 + (tile % ntx) * ti;
+      const int k = 1 + tile / ntx;
+      /* X_1..X_{M-1} as loaded; Z'_0 from X_0 and X_M (element 0 of the threads with r0 = 0) */
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++){
+        V v = pre[q];
+        if(q == 0 && r0 == 0){
+          V xm = prexm;
+          v.y = T(0.);
+          xm.y = T(0.);
+          v = c2r_pre(v, xm, twnl[0]);
+        }
+        sm[t0 * A.ld + fpad(r0 + q * dr)] = v;
+      }
+      fft_lds_barrier();
+      {
+        const int nxt = tile + (int)gridDim.x < ntiles ? tile + (int)gridDim.x : tile;
+        const __amdgpu_buffer_rsrc_t rs = YX_RS(nxt);
+        #pragma unroll
+        for(int q = 0; q < EPTK; q++) pre[q] = tck_bld2(rs, vs, ss * q);
+        prexm = tck_bld2(rs, vxm, 0u);
+      }
+      /* Z'_k1, Z'_k2 for k1 = 1..M/2 (ti M / 2 = blockDim.x EPTK / 2 pairs) */
+      #pragma unroll
+      for(int q = 0; q < EPTK / 2; q++){
+        const int e = threadIdx.x + q * nt;
+        const int t = (e & (ti - 1));
+        const int k1 = 1 + (e >> fft_lg(ti));
+        const int k2 = M - k1;
+        const V xa = sm[t * A.ld + fpad(k1)];
+        const V xb = sm[t * A.ld + fpad(k2)];
+        const V z1 = c2r_pre(xa, xb, twnl[k1]);
+        if(k2 != k1){
+          const V z2 = c2r_pre(xb, xa, twnl[k2]);
+          sm[t * A.ld + fpad(k2)] = z2;
+        }
+        sm[t * A.ld + fpad(k1)] = z1;
+      }
+      fft_lds_barrier();
+      lds_fft<T, EPTK, W>(sm, A.ld, ti, A.P, twl, T(1.));
+      /* element q of this thread: line t0, row jj = r0 + q dr; the interior values first, all
+       * unconditional, then the halo images and wall copies of the threads that have any */
+      const int i = i0 + t0;
+      const T * ln = (const T *)(sm + t0 * A.ld);
+      {
+        const __amdgpu_buffer_rsrc_t rp = tck_rsrc(psi + tck_idx(g, i0, 1, k), pbytes);
+        #pragma unroll
+        for(int q = 0; q < NST; q++){
+          const int jj = r0 + q * dr;
+          tck_bsts(rp, vp, sp * q, ln[2 * fpad(jj >> 1) + (jj & 1)]);
+        }
+      }
+      if(i == 1 || i == g.nx || k == 1 || k == g.nz || r0 == 0 || r0 == dr - 1){
+        for(int q = 0; q < NST; q++){
+          const int jj = r0 + q * dr;
+          tck_store_halo<T, false>(psi, g, i, jj + 1, k, ln[2 * fpad(jj >> 1) + (jj & 1)], true);
+        }
+      }
+      fft_lds_barrier();
+      tile += (int)gridDim.x;
+      if(tile >= ntiles) break;
+    }
+    #undef YX_RS
+    return;
+  }
   int t0 = t0i, r0 = r0i;
   V pre[PF];
   bool pok[PF];
@@ -1143,12 +1247,11 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
 #if TCK_ZFFT_PF > 0
   {
     const long long el = (long long)A.tz * nz;
-    /* the exact form when the tiles cover the spectrum exactly and its twiddle factors fit
-     * in the LDS beside the tile */
+    /* the exact form when the tiles cover the spectrum exactly, the plan has no radix 5 or 7
+     * passes and its twiddle factors fit in the LDS beside the tile */
     const size_t ldsx = lds + cs * (size_t)nz;
-    const bool x = exact && nx % A.tz == 0 && (long long)nt * PERSIST_EPT == el && ldsx <= lds_limit();
-    const int r = mixed ? 0 : x ? (plan_wide(&A.P) ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, true, true>, A, nt, ldsx, el, ntiles, stream)
-                                       : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream))
+    const bool x = exact && !plan_wide(&A.P) && nx % A.tz == 0 && (long long)nt * PERSIST_EPT == el && ldsx <= lds_limit();
+    const int r = mixed ? 0 : x ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
                     : (plan_wide(&A.P) ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, true, false>, A, nt, lds, el, ntiles, stream)
                                        : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
@@ -1162,7 +1265,7 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
 }
 
 int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const void * twM,
-    const void * twN, int mixed, hipStream_t stream){
+    const void * twN, int mixed, int exact, hipStream_t stream){
   YInvArgs A;
   A.g = *g; A.S = S; A.psi = psi; A.nky = g->ny / 2 + 1;
   if(make_plan(g->ny / 2, &A.P) != 0) return (int)hipErrorInvalidValue;
@@ -1177,8 +1280,15 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
 #if TCK_YIFFT_PF > 0
   {
     const long long el = (long long)A.ti * A.P.M;
-    const int r = mixed ? 0 : (plan_wide(&A.P) ? launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, true>, A, nt, lds, el, ntiles, stream)
-                                   : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, false>, A, nt, lds, el, ntiles, stream));
+    /* the exact form when the tiles cover the field exactly, the plan has no radix 5 or 7
+     * passes, both twiddle tables fit in the LDS beside the tile and a plane's byte offsets fit
+     * in 31 bits */
+    const size_t ldsx = lds + cs * (size_t)(2 * A.P.M);
+    const bool x = exact && !plan_wide(&A.P) && g->nx % A.ti == 0 && (long long)nt * PERSIST_EPT == el && ldsx <= lds_limit()
+        && 16LL * g->plane < (1LL << 31) && 16LL * g->nx * A.nky < (1LL << 31);
+    const int r = mixed ? 0 : x ? launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
+                    : (plan_wide(&A.P) ? launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, true, false>, A, nt, lds, el, ntiles, stream)
+                                       : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, false, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }

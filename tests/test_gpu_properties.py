@@ -136,10 +136,11 @@ def test_device_manufactured_potential_long_lines(need_gpu):
 
 
 def test_device_exact_transform_form_computes_the_same_bits(need_gpu):
-    """at 128 x 256 x 1024 the persistent axial transforms take their exact-tiling form (tiles
-    of 8 lines cover nx = 128, 1024 threads x 8 elements cover nz = 1024, twiddle factors in
-    LDS, the next tile loaded in full before the current one's passes); the general persistent
-    form computes the same potential bit for bit"""
+    """at 128 x 256 x 1024 the persistent axial transforms (tiles of 8 lines cover nx = 128,
+    1024 threads x 8 elements cover nz = 1024) and inverse theta transform (tiles of 16 lines,
+    256 threads x 8 elements cover ny / 2 = 128) take their exact-tiling forms (twiddle factors
+    in LDS, the next tile loaded in full before the current one's passes); the general
+    persistent forms compute the same potential bit for bit, each form alone and both"""
     from simpletcsolver_amd import Solver
     nx, ny, nz = 128, 256, 1024
     small = make_case(nx=nx, ny=ny, nz=4, Re=100.0, amp=0.0, uy_walls=(0.0, 0.0))
@@ -149,15 +150,16 @@ def test_device_exact_transform_form_computes_the_same_bits(need_gpu):
     f["ux"][:, :, 0] = 0.0
     f["ux"][:, :, -1] = 0.0
     out = []
-    for exact in (1, 0):
-        s = Solver(nx, ny, nz, small["xf"], small["xc"], **small["kw"])
-        s.tune("fft_exact", exact, 0)
-        s.load_fields(f)
+    s = Solver(nx, ny, nz, small["xf"], small["xc"], **small["kw"])
+    s.load_fields(f)
+    for mask in (0, 1, 4, 5, 5):
+        s.tune("fft_exact", mask, 0)
         s.potential(1, 0.13)
         out.append(s.get_field("psi"))
-        s.close()
-    assert np.array_equal(out[0], out[1])
+    s.close()
     assert np.abs(out[0]).max() > 0.0
+    for n in range(1, len(out)):
+        assert np.array_equal(out[n], out[0]), n
 
 
 @pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 1, 1)])
