@@ -35,8 +35,9 @@ int tck_launch_factor(const double * dt, const int dirs[3], int nx, int ny, int 
     const double * const (*lx)[3], const double * const (*ly)[3],
     double * const (*xt)[3], double * const (*yt)[4], double * const (*zt)[4], hipStream_t stream);
 /* the x solves of nf (1..4) fields in one launch; per field: d the increment (float when
- * mixed), uout where the final pass (ucur non-null) writes ucur + solution (FP64; may be d
- * itself when not mixed), the factor rows r, v, l and the line i0 .. i0+n-1 */
+ * mixed), uout where the final pass (ucur non-null, for every field or for none) writes
+ * ucur + solution (FP64; may be d itself when not mixed), the factor rows r, v, l and the
+ * line i0 .. i0+n-1 */
 int tck_launch_xsolve(const tc_geom_t * g, int nf, void * const * d, double * const * uout,
     const double * const * ucur, const double * const * r, const double * const * v,
     const double * const * l, const int * i0, const int * n, int mixed, hipStream_t stream);

@@ -102,6 +102,14 @@ __device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i
   }
 }
 
+/* a value that is the same in every lane, as a scalar (so that a buffer resource built from it
+ * needs no per-lane loop) */
+__device__ __forceinline__ long long tck_uni64(long long x){
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(unsigned long long)x);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)x >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
 /* Raw buffer access: a resource (base and size in bytes, in scalar registers, uniform across
  * the wave) plus a 32-bit per-lane byte offset and a uniform one, so that an access costs one
  * vector register of address.  Word 3 of the resource is the untyped-access value of gfx9
@@ -110,6 +118,15 @@ __device__ __forceinline__ void tck_store_halo(T * a, const tc_geom_t & g, int i
 typedef unsigned int tck_u2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t tck_rsrc(const void * base, unsigned bytes){
   return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, TCK_BUF_WORD3);
+}
+/* the same for a base and size that are the same in every lane but that the compiler may
+ * not know to be (read from the first lane, so that the resource is built in scalar
+ * registers and an access through it needs no per-lane loop) */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tck_rsrc_u(const void * base, unsigned bytes){
+  const unsigned long long a = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  return tck_rsrc((const void *)(((unsigned long long)hi << 32) | lo), __builtin_amdgcn_readfirstlane(bytes));
 }
 template <typename T> __device__ __forceinline__ T tck_bld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so);
 template <> __device__ __forceinline__ double tck_bld<double>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
@@ -122,9 +139,21 @@ template <> __device__ __forceinline__ float tck_bld<float>(__amdgpu_buffer_rsrc
 __device__ __forceinline__ void tck_bst(__amdgpu_buffer_rsrc_t r, unsigned vo, double v){
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, 0, TCK_NT_STORE ? 2 : 0);
 }
-/* the same with a uniform byte offset so added to vo */
-__device__ __forceinline__ void tck_bsts(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, double v){
+/* load of a streamed operand (cache policy as tck_ld) and store of a streamed result (as
+ * tck_st), FP64 or FP32, at byte offset vo + so of resource r */
+template <typename T> __device__ __forceinline__ T tck_bldn(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ double tck_bldn<double>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, TCK_NT_LOAD ? 2 : 0));
+}
+template <> __device__ __forceinline__ float tck_bldn<float>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, TCK_NT_LOAD ? 2 : 0));
+}
+template <typename T> __device__ __forceinline__ void tck_bstn(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, T v);
+template <> __device__ __forceinline__ void tck_bstn<double>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, double v){
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, so, TCK_NT_STORE ? 2 : 0);
+}
+template <> __device__ __forceinline__ void tck_bstn<float>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, float v){
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, vo, so, TCK_NT_STORE ? 2 : 0);
 }
 /* complex FP64 value: load of a streamed operand (cache policy as tck_ld) at byte offset
  * vo + so of resource r */

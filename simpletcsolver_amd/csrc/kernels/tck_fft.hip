@@ -902,7 +902,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
         #pragma unroll
         for(int q = 0; q < NST; q++){
           const int jj = r0 + q * dr;
-          tck_bsts(rp, vp, sp * q, ln[2 * fpad(jj >> 1) + (jj & 1)]);
+          tck_bstn<double>(rp, vp, sp * q, ln[2 * fpad(jj >> 1) + (jj & 1)]);
         }
       }
       if(i == 1 || i == g.nx || k == 1 || k == g.nz || r0 == 0 || r0 == dr - 1){

@@ -402,7 +402,7 @@ __device__ __forceinline__ void wave_sync(){
  * whole loop they would cost 3 SEG doubles per lane and occupancy.  Lines are staged through
  * a wave-private LDS area: global loads and stores are contiguous across the wave (lane t
  * touches elements t, t+64, ...), each lane solves its contiguous segment. */
-template <int SEG, typename D>
+template <int SEG, typename D, bool U>
 __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict__ dd,
     double * uout, const double * __restrict__ ucur, const double * __restrict__ rt,
     const double * __restrict__ vt, const double * __restrict__ lt, const int i0, const int n,
@@ -420,34 +420,46 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
   const long long nlines = (long long)g.ny * g.nz;
   const int wpb = TCK_BLOCK / TCK_WAVE;
   const long long stride = (long long)gridDim.x * wpb;
-  /* the line's increment (and old field) are loaded into registers one line ahead, so the
-   * loads of the next line are in flight while this one is solved */
+  /* the wave's index as a uniform value: the line, and so each buffer resource, is the wave's */
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  /* Buffer addressing: a line is n contiguous values from cell (i0, j, k), so a load at or past
+   * n reads 0 and a store there writes nothing, and no access is conditional (a wave past the
+   * last line loads and stores through a zero-size resource).  The line's increment (U: and
+   * old field) are loaded into registers one line ahead, before this line's stores, so the
+   * loads of the next line are in flight while this one is solved; outstanding loads and
+   * stores complete in issue order as far as the wait counter is concerned, and the zero-size
+   * stores before the loop give its entry the loads-then-stores sequence of an iteration's
+   * end, so the put of a line waits for its loads and not for the stores before them. */
+  const unsigned nbd = (unsigned)n * (unsigned)sizeof(D), nbu = (unsigned)n * 8u;
+  const unsigned vod = (unsigned)lane * (unsigned)sizeof(D), vou = (unsigned)lane * 8u;
   D pd[SEG];
   double pu[SEG];
   #define XS_FETCH(ln) { \
     const long long l_ = (ln); \
-    if(l_ < nlines){ \
-      const long long b_ = tck_idx(g, i0, 1 + (int)(l_ % g.ny), 1 + (int)(l_ / g.ny)); \
-      _Pragma("unroll") for(int e = 0; e < SEG; e++){ \
-        const int idx = lane + e * TCK_WAVE; \
-        pd[e] = idx < n ? tck_ld(&dd[b_ + idx]) : D(0.); \
-        if(ucur) pu[e] = idx < n ? tck_ld(&ucur[b_ + idx]) : 0.; \
-      } \
+    const bool in_ = l_ < nlines; \
+    const long long b_ = in_ ? tck_idx(g, i0, 1 + (int)(l_ % g.ny), 1 + (int)(l_ / g.ny)) : 0; \
+    const __amdgpu_buffer_rsrc_t rd_ = tck_rsrc_u(dd + b_, in_ ? nbd : 0u); \
+    _Pragma("unroll") for(int e = 0; e < SEG; e++) pd[e] = tck_bldn<D>(rd_, vod, (unsigned)(e * TCK_WAVE * (int)sizeof(D))); \
+    if constexpr (U){ \
+      const __amdgpu_buffer_rsrc_t ru_ = tck_rsrc_u(ucur + b_, in_ ? nbu : 0u); \
+      _Pragma("unroll") for(int e = 0; e < SEG; e++) pu[e] = tck_bldn<double>(ru_, vou, (unsigned)(e * TCK_WAVE * 8)); \
     } }
-  XS_FETCH((long long)blockIdx.x * wpb + w);
+  XS_FETCH((long long)blockIdx.x * wpb + wu);
+  {
+    const __amdgpu_buffer_rsrc_t nul = tck_rsrc(dd, 0u);
+    #pragma unroll
+    for(int e = 0; e < SEG; e++) tck_bst0(nul, 16u * e);
+  }
   for(long long g0 = (long long)blockIdx.x * wpb; g0 < nlines; g0 += stride){
-    const long long line = g0 + w;
+    const long long line = g0 + wu;
     const bool live = line < nlines;
     const int j = live ? 1 + (int)(line % g.ny) : 1;
     const int k = live ? 1 + (int)(line / g.ny) : 1;
     double u[SEG];
-    if(live){
-      #pragma unroll
-      for(int e = 0; e < SEG; e++){
-        const int idx = lane + e * TCK_WAVE;
-        if(idx < n) sm[stage_pos<SEG>(idx)] = double(pd[e]);
-        u[e] = pu[e];
-      }
+    #pragma unroll
+    for(int e = 0; e < SEG; e++){
+      sm[stage_pos<SEG>(lane + e * TCK_WAVE)] = double(pd[e]);
+      if constexpr (U) u[e] = pu[e];
     }
     XS_FETCH(line + stride);
     const long long base = tck_idx(g, i0, j, k);
@@ -462,32 +474,36 @@ __device__ __forceinline__ void xsolve_body(const XSolveArgs & A, D * __restrict
     #pragma unroll
     for(int e = 0; e < SEG; e++) sm[lane * (SEG + 1) + e] = q[e][0];
     wave_sync();
-    if(live){
-      if(ucur){
+    if constexpr (U){
+      const __amdgpu_buffer_rsrc_t ro = tck_rsrc_u(uout + base, live ? nbu : 0u);
+      #pragma unroll
+      for(int e = 0; e < SEG; e++)
+        tck_bstn<double>(ro, vou, (unsigned)(e * TCK_WAVE * 8), u[e] + sm[stage_pos<SEG>(lane + e * TCK_WAVE)]);
+      /* the halo images of the lines next to the periodic boundaries */
+      if(live && (j == 1 || j == g.ny || k == 1 || k == g.nz)){
         #pragma unroll
         for(int e = 0; e < SEG; e++){
           const int idx = lane + e * TCK_WAVE;
-          if(idx < n) tck_store_halo(uout, g, i0 + idx, j, k, u[e] + sm[stage_pos<SEG>(idx)], false);
-        }
-      }else{
-        #pragma unroll
-        for(int e = 0; e < SEG; e++){
-          const int idx = lane + e * TCK_WAVE;
-          if(idx < n) tck_st(&dd[base + idx], D(sm[stage_pos<SEG>(idx)]));
+          if(idx < n) tck_store_halo<double, false>(uout, g, i0 + idx, j, k, u[e] + sm[stage_pos<SEG>(idx)], false);
         }
       }
+    }else{
+      const __amdgpu_buffer_rsrc_t ro = tck_rsrc_u(dd + base, live ? nbd : 0u);
+      #pragma unroll
+      for(int e = 0; e < SEG; e++)
+        tck_bstn<D>(ro, vod, (unsigned)(e * TCK_WAVE * (int)sizeof(D)), D(sm[stage_pos<SEG>(lane + e * TCK_WAVE)]));
     }
     wave_sync();
   }
   #undef XS_FETCH
 }
 
-template <int SEG, typename D>
-__global__ void __launch_bounds__(TCK_BLOCK) tck_xsolve(const XSolveArgs A){
+template <int SEG, typename D, bool U>
+__global__ void __launch_bounds__(TCK_BLOCK) __attribute__((amdgpu_waves_per_eu(SEG <= 8 ? 4 : 1))) tck_xsolve(const XSolveArgs A){
   __shared__ double xs_sm[TCK_BLOCK / TCK_WAVE][TCK_WAVE * (SEG + 1)];
   __shared__ double xs_tab[3][TCK_WAVE * (SEG + 1)];
   const int f = blockIdx.y;
-  xsolve_body<SEG, D>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.l[f], A.i0[f], A.n[f],
+  xsolve_body<SEG, D, U>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.l[f], A.i0[f], A.n[f],
       xs_sm[threadIdx.x / TCK_WAVE], xs_tab);
 }
 
@@ -909,8 +925,15 @@ static int launch_xsolve_t(const XSolveArgs & A, int nf, int mixed, hipStream_t 
   const long long cap = (long long)num_cus() * 8;
   if(nblocks > cap) nblocks = cap;
   const dim3 grd((unsigned)nblocks, (unsigned)nf);
-  if(mixed) hipLaunchKernelGGL((tck_xsolve<SEG, float>), grd, dim3(TCK_BLOCK), 0, stream, A);
-  else hipLaunchKernelGGL((tck_xsolve<SEG, double>), grd, dim3(TCK_BLOCK), 0, stream, A);
+  /* the final pass (the new fields written from the old ones) is a form of its own */
+  const bool u = A.ucur[0] != NULL;
+  if(mixed){
+    if(u) hipLaunchKernelGGL((tck_xsolve<SEG, float, true>), grd, dim3(TCK_BLOCK), 0, stream, A);
+    else hipLaunchKernelGGL((tck_xsolve<SEG, float, false>), grd, dim3(TCK_BLOCK), 0, stream, A);
+  }else{
+    if(u) hipLaunchKernelGGL((tck_xsolve<SEG, double, true>), grd, dim3(TCK_BLOCK), 0, stream, A);
+    else hipLaunchKernelGGL((tck_xsolve<SEG, double, false>), grd, dim3(TCK_BLOCK), 0, stream, A);
+  }
   TCK_CHECK_LAUNCH();
   return 0;
 }
@@ -968,6 +991,7 @@ int tck_launch_xsolve(const tc_geom_t * g, int nf, void * const * d, double * co
   int nmax = 0;
   for(int f = 0; f < nf; f++){
     if(n[f] > TCK_WAVE * XS_MAXSEG || n[f] < 2) return (int)hipErrorInvalidValue;
+    if((ucur[f] == NULL) != (ucur[0] == NULL)) return (int)hipErrorInvalidValue;   /* all final or none */
     A.d[f] = d[f]; A.uout[f] = uout[f]; A.ucur[f] = ucur[f];
     A.r[f] = r[f]; A.v[f] = v[f]; A.l[f] = l[f]; A.i0[f] = i0[f]; A.n[f] = n[f];
     if(n[f] > nmax) nmax = n[f];
