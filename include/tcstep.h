@@ -116,9 +116,10 @@ int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num)
  * marched per block (1..nz) and ty j rows per block of 64 x ty threads (1, 2 or 4).  Kernel
  * "predict_window" with kb 1 or 0 switches the predictor's form that stages the fields'
  * planes in LDS (32 x 8 tiles, taken when ty is 4; the default) on or off; "fft_exact" with
- * kb a mask (1: the axial transforms, 4: the inverse theta transform, 5: both, the default)
- * the exact-tiling form of those persistent FP64 transforms where the tiles cover the spectrum
- * exactly.  Results are identical for every shape and form; only the speed changes */
+ * kb a mask (1: the axial transforms, 4: the inverse theta transform: their exact-tiling
+ * forms where the tiles cover the spectrum exactly; 2: the divergence + theta transform's
+ * twiddle tables in LDS; 7, the default: all three) the forms of the persistent FP64
+ * transforms.  Results are identical for every shape and form; only the speed changes */
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty);
 /* no reference counterpart: times the predictor for every candidate shape (kb in 2..32,
  * ty in 1, 2, 4) on the solver's own buffers, reps launches each, and keeps the fastest for

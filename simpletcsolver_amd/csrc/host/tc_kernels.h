@@ -58,7 +58,7 @@ int tck_fft_supported(int n);
 /* mixed: spectrum, transforms, twiddles and psi in FP32 */
 int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
     const double * uy, const double * uz, const double * dt, int rkstep, void * S,
-    const void * twM, const void * twN, int mixed, hipStream_t stream);
+    const void * twM, const void * twN, int mixed, int twl, hipStream_t stream);
 int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign, int mixed, int exact, hipStream_t stream);
 int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const void * twM,
     const void * twN, int mixed, int exact, hipStream_t stream);

@@ -362,7 +362,7 @@ static void default_shapes(tc_solver_t * s){
     s->tune[which[q]][1] = ty;
   }
   s->predict_window = 1;
-  s->fft_exact = 5;
+  s->fft_exact = 7;
 }
 
 int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_solver_t ** out){
@@ -864,7 +864,7 @@ static int potential(tc_solver_t * s, int rkstep){
   const int mx = s->mixed;
   void * psi = mx ? (void *)s->d_psi32 : (void *)s->d_psi;
   PROF(TC_S_DIVY, tck_launch_div_yfft(g, &s->coef, tc_field_ptr(s, TC_FUX), tc_field_ptr(s, TC_FUY),
-      tc_field_ptr(s, TC_FUZ), s->d_dt, rkstep, s->d_spec, s->d_twy, s->d_twyn, mx, s->stream));
+      tc_field_ptr(s, TC_FUZ), s->d_dt, rkstep, s->d_spec, s->d_twy, s->d_twyn, mx, s->fft_exact & 2, s->stream));
   PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, -1, mx, s->fft_exact & 1, s->stream));
   PROF(TC_S_PXSOLVE, tck_launch_pxsolve(g->nx, s->nky * g->nz, s->d_spec, s->d_pr, s->d_pl, s->d_pu, mx, s->stream));
   PROF(TC_S_ZFFT, tck_launch_zfft(g->nx, s->nky, g->nz, s->d_spec, s->d_twz, +1, mx, s->fft_exact & 1, s->stream));
@@ -1135,8 +1135,9 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
 
 /* launch shapes of the cell kernels ("predict", "correct"): planes per block and j rows per
  * block; results do not depend on them.  "predict_window" (kb 0 or 1) switches between forms
- * of the predictor, "fft_exact" (kb a mask: 1 the axial transforms, 4 the inverse theta
- * transform) between forms of the persistent transforms; each pair computes the same bits */
+ * of the predictor, "fft_exact" (kb a mask: 1 the axial transforms, 2 the divergence + theta
+ * transform's twiddles in LDS, 4 the inverse theta transform) between forms of the persistent
+ * transforms; each pair computes the same bits */
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
   if(kernel && strcmp(kernel, "predict_window") == 0){
     s->predict_window = kb ? 1 : 0;
@@ -1144,7 +1145,7 @@ int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
     return 0;
   }
   if(kernel && strcmp(kernel, "fft_exact") == 0){
-    s->fft_exact = kb & 5;
+    s->fft_exact = kb & 7;
     s->gen++;
     return 0;
   }

@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft(const YFwdArgs A)
 #define TCK_DIVY_KRUN 1
 #endif
 
-template <typename T, int PF, int EPTK, bool W>
+template <typename T, int PF, int EPTK, bool W, bool TL>
 __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs A){
   typedef typename cvec<T>::type V;
   extern __shared__ __align__(16) unsigned char fft_smem[];
@@ -411,6 +411,16 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
   const long long sz = g.plane;
   const V * twM = (const V *)A.twM;
   const V * twN = (const V *)A.twN;
+  if constexpr (TL){
+    /* TL: both twiddle tables copied to LDS after the tile's lines, so that neither the passes
+     * nor the split below wait for a global load (a wait for a load also waits for the stores
+     * issued before it: in the split, each element's stores would wait for the last's) */
+    V * twl = sm + ti * A.ld;
+    for(int t = threadIdx.x; t < M; t += nt) twl[t] = twM[t];
+    for(int t = threadIdx.x; t <= M; t += nt) twl[M + t] = twN[t];
+    twM = twl;
+    twN = twl + M;
+  }
   V * S = (V *)A.S;
   /* element q of this thread in a tile: line t0, row j = r0 + q dr (the block size is a
    * multiple of the tile width, a power of two no larger than 64) */
@@ -1204,7 +1214,7 @@ extern "C" {
 /* mixed: spectrum, transforms and psi in FP32 (twiddles twM, twN, tw as float2) */
 int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double * ux,
     const double * uy, const double * uz, const double * dt, int rkstep, void * S,
-    const void * twM, const void * twN, int mixed, hipStream_t stream){
+    const void * twM, const void * twN, int mixed, int twl, hipStream_t stream){
   YFwdArgs A;
   A.g = *g; A.c = *c; A.ux = ux; A.uy = uy; A.uz = uz; A.dt = dt; A.rkstep = rkstep;
   A.S = S; A.nky = g->ny / 2 + 1;
@@ -1220,8 +1230,12 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
 #if TCK_DIVY_PF > 0
   {
     const long long el = (long long)A.ti * A.P.M;
-    const int r = mixed ? 0 : (plan_wide(&A.P) ? launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, true>, A, nt, lds, el, ntiles, stream)
-                                   : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, false>, A, nt, lds, el, ntiles, stream));
+    /* the twiddle tables in LDS where they fit beside the tile (radix 2, 3, 4, 8 plans) */
+    const size_t ldsx = lds + cs * (size_t)(2 * A.P.M + 1);
+    const bool tl = twl && !plan_wide(&A.P) && ldsx <= lds_limit();
+    const int r = mixed ? 0 : tl ? launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
+                    : (plan_wide(&A.P) ? launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, true, false>, A, nt, lds, el, ntiles, stream)
+                                       : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, false, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
     if(r > 0){ TCK_CHECK_LAUNCH(); return 0; }
   }
