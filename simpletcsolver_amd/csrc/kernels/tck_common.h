@@ -155,6 +155,14 @@ template <> __device__ __forceinline__ void tck_bstn<double>(__amdgpu_buffer_rsr
 template <> __device__ __forceinline__ void tck_bstn<float>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, float v){
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, vo, so, TCK_NT_STORE ? 2 : 0);
 }
+/* complex FP32 value: load of a streamed operand and store of a streamed result (cache
+ * policies as tck_ld and tck_st) at byte offset vo + so of resource r */
+__device__ __forceinline__ float2 tck_bldf2(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, TCK_NT_LOAD ? 2 : 0));
+}
+__device__ __forceinline__ void tck_bstf2(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, float2 v){
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, so, TCK_NT_STORE ? 2 : 0);
+}
 /* complex FP64 value: load of a streamed operand (cache policy as tck_ld) at byte offset
  * vo + so of resource r */
 typedef unsigned int tck_u4v __attribute__((ext_vector_type(4)));

@@ -618,8 +618,21 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
     int tile = blockIdx.x;             /* the launch has at least four tiles per block */
     V pre[EPTK];
     #define ZX_OFF(tl, q) ((long long)A.nx * ((tl) / ntx) + (long long)(((tl) % ntx) * tz + t0i) + kstride * (long long)(r0i + (q) * dr))
-    #pragma unroll
-    for(int q = 0; q < EPTK; q++) pre[q] = tck_ld((const V *)A.S + ZX_OFF(tile, q));
+    /* FP32: the spectrum spans under 4 GB (checked on the host), so a tile's accesses are a
+     * buffer resource at the tile's first column plus 32-bit offsets: one register of address
+     * per thread instead of one 64-bit address per element */
+    const unsigned vz = (unsigned)sizeof(V) * (unsigned)(t0i + kstride * r0i);
+    const unsigned sq = (unsigned)sizeof(V) * (unsigned)(kstride * dr);
+    #define ZX_RS(tl) tck_rsrc_u((const V *)A.S + (long long)A.nx * ((tl) / ntx) + (long long)(((tl) % ntx) * tz), \
+      (unsigned)sizeof(V) * (unsigned)(kstride * nz))
+    if constexpr (sizeof(T) == 4){
+      const __amdgpu_buffer_rsrc_t rs = ZX_RS(tile);
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++) pre[q] = tck_bldf2(rs, vz, sq * q);
+    }else{
+      #pragma unroll
+      for(int q = 0; q < EPTK; q++) pre[q] = tck_ld((const V *)A.S + ZX_OFF(tile, q));
+    }
     {
       const __amdgpu_buffer_rsrc_t nul = tck_rsrc(A.S, 0u);
       #pragma unroll
@@ -630,20 +643,33 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_zfft_p(const ZArgs A){
       for(int q = 0; q < EPTK; q++) sm[t0i * A.ld + fpad(r0i + q * dr)] = pre[q];
       fft_lds_barrier();
       const int nxt = tile + (int)gridDim.x < ntiles ? tile + (int)gridDim.x : tile;
-      #pragma unroll
-      for(int q = 0; q < EPTK; q++) pre[q] = tck_ld((const V *)A.S + ZX_OFF(nxt, q));
+      if constexpr (sizeof(T) == 4){
+        const __amdgpu_buffer_rsrc_t rs = ZX_RS(nxt);
+        #pragma unroll
+        for(int q = 0; q < EPTK; q++) pre[q] = tck_bldf2(rs, vz, sq * q);
+      }else{
+        #pragma unroll
+        for(int q = 0; q < EPTK; q++) pre[q] = tck_ld((const V *)A.S + ZX_OFF(nxt, q));
+      }
       lds_fft<T, EPTK, W>(sm, A.ld, tz, A.P, twl, T(A.sign));
-      V * base = (V *)A.S + (long long)A.nx * (tile / ntx) + (long long)((tile % ntx) * tz + t0i);
-      #pragma unroll
-      for(int q = 0; q < EPTK; q++){
-        const int kz = r0i + q * dr;
-        tck_st(&base[kstride * kz], sm[t0i * A.ld + fpad(kz)]);
+      if constexpr (sizeof(T) == 4){
+        const __amdgpu_buffer_rsrc_t rs = ZX_RS(tile);
+        #pragma unroll
+        for(int q = 0; q < EPTK; q++) tck_bstf2(rs, vz, sq * q, sm[t0i * A.ld + fpad(r0i + q * dr)]);
+      }else{
+        V * base = (V *)A.S + (long long)A.nx * (tile / ntx) + (long long)((tile % ntx) * tz + t0i);
+        #pragma unroll
+        for(int q = 0; q < EPTK; q++){
+          const int kz = r0i + q * dr;
+          tck_st(&base[kstride * kz], sm[t0i * A.ld + fpad(kz)]);
+        }
       }
       fft_lds_barrier();
       tile += (int)gridDim.x;
       if(tile >= ntiles) break;
     }
     #undef ZX_OFF
+    #undef ZX_RS
     return;
   }
   int t0 = t0i, r0 = r0i;
@@ -1181,8 +1207,8 @@ static int fft_one(const void * fn, F launch, size_t lds){
 
 template <typename Args>
 static int launch_persistent(void (*kern)(const Args), const Args & A, int nt, size_t lds,
-    long long elems, long long ntiles, hipStream_t stream){
-  if((long long)nt * PERSIST_EPT < elems || nt > FFT_MAXTHREADS) return 0;
+    long long elems, long long ntiles, hipStream_t stream, int ept = PERSIST_EPT){
+  if((long long)nt * ept < elems || nt > FFT_MAXTHREADS) return 0;
   const void * fn = (const void *)kern;
   int per = 0;
   for(int q = 0; q < g_pc_n; q++) if(g_pc_fn[q] == fn && g_pc_nt[q] == nt && g_pc_lds[q] == lds) per = g_pc_per[q];
@@ -1265,7 +1291,12 @@ int tck_launch_zfft(int nx, int nky, int nz, void * S, const void * tw, int sign
      * passes and its twiddle factors fit in the LDS beside the tile */
     const size_t ldsx = lds + cs * (size_t)nz;
     const bool x = exact && !plan_wide(&A.P) && nx % A.tz == 0 && (long long)nt * PERSIST_EPT == el && ldsx <= lds_limit();
-    const int r = mixed ? 0 : x ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
+    /* FP32 tiles (twice as wide, FFT_EPT elements per thread): only the exact form is
+     * persistent, its registers holding the next tile in place of the prefetch */
+    const bool xf = exact && !plan_wide(&A.P) && nx % A.tz == 0 && (long long)nt * FFT_EPT == el && ldsx <= lds_limit()
+        && (long long)sizeof(float2) * nx * nky * nz < (1LL << 32);
+    const int r = mixed ? (xf ? launch_persistent(tck_zfft_p<float, TCK_ZFFT_PF, FFT_EPT, false, true>, A, nt, ldsx, el, ntiles, stream, FFT_EPT) : 0)
+                : x ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
                     : (plan_wide(&A.P) ? launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, true, false>, A, nt, lds, el, ntiles, stream)
                                        : launch_persistent(tck_zfft_p<double, TCK_ZFFT_PF, PERSIST_EPT, false, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;

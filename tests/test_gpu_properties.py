@@ -163,6 +163,32 @@ def test_device_exact_transform_form_computes_the_same_bits(need_gpu):
         assert np.array_equal(out[n], out[0]), n
 
 
+def test_device_exact_fp32_axial_transform_computes_the_same_bits(need_gpu):
+    """the mixed step's axial transforms at 128 x 256 x 1024: FP32 tiles of 16 lines, 1024
+    threads x 16 elements covering nz = 1024, take the persistent exact-tiling form (twiddles in
+    LDS, buffer-addressed, the whole next tile loaded before the passes) where the one-tile
+    form ran before; both compute the same potential bit for bit"""
+    from simpletcsolver_amd import Solver
+    nx, ny, nz = 128, 256, 1024
+    small = make_case(nx=nx, ny=ny, nz=4, Re=100.0, amp=0.0, uy_walls=(0.0, 0.0))
+    rng = np.random.default_rng(43)
+    f = {"ux": rng.standard_normal((nz, ny, nx + 1)), "uy": rng.standard_normal((nz, ny, nx + 2)),
+         "uz": rng.standard_normal((nz, ny, nx + 2))}
+    f["ux"][:, :, 0] = 0.0
+    f["ux"][:, :, -1] = 0.0
+    out = []
+    s = Solver(nx, ny, nz, small["xf"], small["xc"], precision="mixed", **small["kw"])
+    s.load_fields(f)
+    for mask in (0, 1, 7, 7):
+        s.tune("fft_exact", mask, 0)
+        s.potential(1, 0.13)
+        out.append(s.get_field("psi"))
+    s.close()
+    assert np.abs(out[0]).max() > 0.0
+    for n in range(1, len(out)):
+        assert np.array_equal(out[n], out[0]), n
+
+
 @pytest.mark.parametrize("implicit", [(0, 0, 0), (1, 1, 1)])
 def test_device_step_is_solenoidal(need_gpu, implicit):
     case = make_case(nx=24, ny=16, nz=12, Re=300.0, implicit=implicit, amp=0.3)
