@@ -169,6 +169,10 @@ typedef unsigned int tck_u4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ double2 tck_bld2(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){
   return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, TCK_NT_LOAD ? 2 : 0));
 }
+/* the complex load of either precision */
+template <typename V> __device__ __forceinline__ V tck_bldc(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ double2 tck_bldc<double2>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){ return tck_bld2(r, vo, so); }
+template <> __device__ __forceinline__ float2 tck_bldc<float2>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so){ return tck_bldf2(r, vo, so); }
 /* 16-byte store of zeros at byte offset vo of resource r; with a zero-size resource it writes
  * nothing but is counted among the outstanding memory operations like any store (distinct
  * offsets keep the compiler from merging such stores) */

@@ -861,26 +861,26 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
     V * twnl = twl + M;
     const V * twM = (const V *)A.twM;
     for(int t = threadIdx.x; t < M; t += nt){ twl[t] = twM[t]; twnl[t] = twN[t]; }
-    static_assert(sizeof(T) == 8, "the exact form is compiled for FP64 tiles");
     constexpr int NST = 2 * EPTK;        /* psi values stored per thread: ti ny = 2 blockDim.x EPTK */
     int tile = blockIdx.x;               /* the launch has at least four tiles per block */
     /* buffer addressing: a tile's spectrum values lie in one plane of S (nky nx complex values)
      * and its psi values in one plane of the field, so each access is a per-thread byte offset
      * that does not change between tiles plus a uniform one */
-    const unsigned vs = 16u * (unsigned)(t0i + g.nx * r0i);            /* X_kk, kk = r0 + q dr */
-    const unsigned ss = 16u * (unsigned)(g.nx * dr);                   /* per q */
-    const unsigned vxm = 16u * (unsigned)(t0i + g.nx * M);             /* X_M */
-    const unsigned sbytes = 16u * (unsigned)(g.nx * A.nky);
-    const unsigned vp = 8u * (unsigned)(t0i + g.px * r0i);             /* psi row r0 + 1 + q dr */
-    const unsigned sp = 8u * (unsigned)(g.px * dr);
-    const unsigned pbytes = 8u * (unsigned)(g.px * g.ny);
+    constexpr unsigned CB = sizeof(V), RB = sizeof(T);                 /* complex, real bytes */
+    const unsigned vs = CB * (unsigned)(t0i + g.nx * r0i);             /* X_kk, kk = r0 + q dr */
+    const unsigned ss = CB * (unsigned)(g.nx * dr);                    /* per q */
+    const unsigned vxm = CB * (unsigned)(t0i + g.nx * M);              /* X_M */
+    const unsigned sbytes = CB * (unsigned)(g.nx * A.nky);
+    const unsigned vp = RB * (unsigned)(t0i + g.px * r0i);             /* psi row r0 + 1 + q dr */
+    const unsigned sp = RB * (unsigned)(g.px * dr);
+    const unsigned pbytes = RB * (unsigned)(g.px * g.ny);
     #define YX_RS(tl) tck_rsrc(S + (long long)((tl) % ntx * ti) + (long long)g.nx * A.nky * ((tl) / ntx), sbytes)
     V pre[EPTK], prexm;
     {
       const __amdgpu_buffer_rsrc_t rs = YX_RS(tile);
       #pragma unroll
-      for(int q = 0; q < EPTK; q++) pre[q] = tck_bld2(rs, vs, ss * q);
-      prexm = tck_bld2(rs, vxm, 0u);
+      for(int q = 0; q < EPTK; q++) pre[q] = tck_bldc<V>(rs, vs, ss * q);
+      prexm = tck_bldc<V>(rs, vxm, 0u);
       const __amdgpu_buffer_rsrc_t nul = tck_rsrc(A.S, 0u);
       #pragma unroll
       for(int q = 0; q < NST; q++) tck_bst0(nul, 16u * q);
@@ -908,8 +908,8 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
         const int nxt = tile + (int)gridDim.x < ntiles ? tile + (int)gridDim.x : tile;
         const __amdgpu_buffer_rsrc_t rs = YX_RS(nxt);
         #pragma unroll
-        for(int q = 0; q < EPTK; q++) pre[q] = tck_bld2(rs, vs, ss * q);
-        prexm = tck_bld2(rs, vxm, 0u);
+        for(int q = 0; q < EPTK; q++) pre[q] = tck_bldc<V>(rs, vs, ss * q);
+        prexm = tck_bldc<V>(rs, vxm, 0u);
       }
       /* Z'_k1, Z'_k2 for k1 = 1..M/2 (ti M / 2 = blockDim.x EPTK / 2 pairs) */
       #pragma unroll
@@ -938,7 +938,7 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_yifft_psi_p(const YInvArgs
         #pragma unroll
         for(int q = 0; q < NST; q++){
           const int jj = r0 + q * dr;
-          tck_bstn<double>(rp, vp, sp * q, ln[2 * fpad(jj >> 1) + (jj & 1)]);
+          tck_bstn<T>(rp, vp, sp * q, ln[2 * fpad(jj >> 1) + (jj & 1)]);
         }
       }
       if(i == 1 || i == g.nx || k == 1 || k == g.nz || r0 == 0 || r0 == dr - 1){
@@ -1331,7 +1331,16 @@ int tck_launch_yifft_psi(const tc_geom_t * g, const void * S, void * psi, const 
     const size_t ldsx = lds + cs * (size_t)(2 * A.P.M);
     const bool x = exact && !plan_wide(&A.P) && g->nx % A.ti == 0 && (long long)nt * PERSIST_EPT == el && ldsx <= lds_limit()
         && 16LL * g->plane < (1LL << 31) && 16LL * g->nx * A.nky < (1LL << 31);
-    const int r = mixed ? 0 : x ? launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
+    /* FP32 tiles: only the exact form is persistent, with FFT_EPT / 2 elements per thread
+     * (twice the threads of the one-tile form), which keeps its registers below 128 */
+    const int ntf = (int)(el / PERSIST_EPT);
+    YInvArgs AF = A;
+    AF.P.nt = ntf;
+    const bool xf = exact && !plan_wide(&A.P) && g->nx % A.ti == 0 && ntf <= FFT_MAXTHREADS && ntf % 64 == 0
+        && ntf % A.ti == 0 && (long long)ntf * PERSIST_EPT == el && ldsx <= lds_limit()
+        && 16LL * g->plane < (1LL << 31) && 16LL * g->nx * A.nky < (1LL << 31);
+    const int r = mixed ? (xf ? launch_persistent(tck_yifft_psi_p<float, TCK_YIFFT_PF, PERSIST_EPT, false, true>, AF, ntf, ldsx, el, ntiles, stream) : 0)
+                : x ? launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
                     : (plan_wide(&A.P) ? launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, true, false>, A, nt, lds, el, ntiles, stream)
                                        : launch_persistent(tck_yifft_psi_p<double, TCK_YIFFT_PF, PERSIST_EPT, false, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;
