@@ -443,6 +443,18 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
   constexpr int EPT2 = 2 * EPTK;
   constexpr int G = 2;
 #if TCK_DIVY_KRUN
+  /* The same with uz's lower face taken from uzc when cz: in a block's run of planes the next
+   * tile is the same column one plane up, whose lower faces are this tile's upper faces, held
+   * in registers (the same values, so the same bits) instead of read again */
+  double uzc[EPT2];
+  #define DV_LOADC(tl, q, P, ok, cz) { \
+    const int j_ = r0 + (q) * dr; \
+    const int i_ = 1 + ((tl) % ntx) * ti + t0; \
+    ok = j_ < g.ny && i_ <= g.nx; \
+    const long long o_ = tck_idx(g, ok ? i_ : 1, ok ? j_ + 1 : 1, 1 + (tl) / ntx); \
+    P[0] = A.ux[o_]; P[1] = A.ux[o_ + 1]; P[2] = A.uy[o_]; P[3] = A.uy[o_ + sy]; \
+    if(cz) P[4] = uzc[q]; else P[4] = A.uz[o_]; \
+    P[5] = A.uz[o_ + sz]; }
   /* each block walks a contiguous run of planes of one column of tiles, so that the uz
    * plane a tile reads as its upper face is read again as the next tile's lower face soon
    * after, from cache; tile index tile = tx + ntx (k - 1) as in the plain order */
@@ -457,6 +469,11 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
   for(; cc < c1; cc++){
     const int tile = DV_TILE(cc);
     const int tnext = cc + 1 < c1 ? DV_TILE(cc + 1) : ntiles;
+    /* this tile's lower faces in uzc (the previous tile was the plane below), and the next
+     * tile's (it is the plane above); not with radix 5 and 7 plans, whose passes need the
+     * registers */
+    const bool cz = !W && cc > c0 && tile == DV_TILE(cc - 1) + ntx;
+    const bool czn = !W && tnext == tile + ntx;
 #else
   int tile = blockIdx.x;
   if(tile < ntiles){
@@ -492,19 +509,39 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
     for(int q0 = PF; q0 < EPT2; q0 += G){
       double P[G][6];
       bool ok[G];
+#if TCK_DIVY_KRUN
+      #pragma unroll
+      for(int u = 0; u < G; u++) if(q0 + u < EPT2){ DV_LOADC(tile, q0 + u, P[u], ok[u], cz); }
+#else
       #pragma unroll
       for(int u = 0; u < G; u++) if(q0 + u < EPT2){ DV_LOAD(tile, q0 + u, P[u], ok[u]); }
+#endif
       if(q0 == PF){
         #pragma unroll
-        for(int q = 0; q < PF; q++) put(q, pre[q], pok[q]);
+        for(int q = 0; q < PF; q++){
+          put(q, pre[q], pok[q]);
+#if TCK_DIVY_KRUN
+          uzc[q] = pre[q][5];
+#endif
+        }
       }
       #pragma unroll
-      for(int u = 0; u < G; u++) if(q0 + u < EPT2) put(q0 + u, P[u], ok[u]);
+      for(int u = 0; u < G; u++) if(q0 + u < EPT2){
+        put(q0 + u, P[u], ok[u]);
+#if TCK_DIVY_KRUN
+        uzc[q0 + u] = P[u][5];
+#endif
+      }
     }
     fft_lds_barrier();
     if(tnext < ntiles){
+#if TCK_DIVY_KRUN
+      #pragma unroll
+      for(int q = 0; q < PF; q++){ DV_LOADC(tnext, q, pre[q], pok[q], czn); }
+#else
       #pragma unroll
       for(int q = 0; q < PF; q++){ DV_LOAD(tnext, q, pre[q], pok[q]); }
+#endif
     }
     lds_fft<T, EPTK, W>(sm, A.ld, ti, A.P, twM, T(-1.));
     for(int e = threadIdx.x; e < ti * (M + 1); e += nt){
@@ -524,6 +561,9 @@ __global__ void __launch_bounds__(FFT_MAXTHREADS) tck_div_yfft_p(const YFwdArgs 
     fft_lds_barrier();
   }
   #undef DV_LOAD
+#if TCK_DIVY_KRUN
+  #undef DV_LOADC
+#endif
 #if TCK_DIVY_KRUN
   #undef DV_TILE
 #endif
