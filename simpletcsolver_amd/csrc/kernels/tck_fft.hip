@@ -1299,9 +1299,15 @@ int tck_launch_div_yfft(const tc_geom_t * g, const tc_coef_t * c, const double *
     /* the twiddle tables in LDS where they fit beside the tile (radix 2, 3, 4, 8 plans) */
     const size_t ldsx = lds + cs * (size_t)(2 * A.P.M + 1);
     const bool tl = twl && !plan_wide(&A.P) && ldsx <= lds_limit();
-    /* (FP32 tiles stay with the one-tile form: a persistent FP32 form of this kernel at 8
-     * elements per thread was 2 % slower) */
-    const int r = mixed ? 0 : tl ? launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
+    /* FP32 tiles: persistent only in this form, with FFT_EPT / 2 elements per thread (twice
+     * the threads of the one-tile form), which keeps the carried faces and the prefetch in
+     * registers */
+    const int ntf = (int)(el / PERSIST_EPT);
+    YFwdArgs AF = A;
+    AF.P.nt = ntf;
+    const bool tlf = tl && ntf <= FFT_MAXTHREADS && ntf % 64 == 0 && ntf % A.ti == 0 && (long long)ntf * PERSIST_EPT == el;
+    const int r = mixed ? (tlf ? launch_persistent(tck_div_yfft_p<float, TCK_DIVY_PF, PERSIST_EPT, false, true>, AF, ntf, ldsx, el, ntiles, stream) : 0)
+                : tl ? launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, false, true>, A, nt, ldsx, el, ntiles, stream)
                     : (plan_wide(&A.P) ? launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, true, false>, A, nt, lds, el, ntiles, stream)
                                        : launch_persistent(tck_div_yfft_p<double, TCK_DIVY_PF, PERSIST_EPT, false, false>, A, nt, lds, el, ntiles, stream));
     if(r < 0) return (int)hipErrorInvalidValue;

@@ -167,9 +167,10 @@ def test_device_exact_fp32_transforms_compute_the_same_bits(need_gpu):
     """the mixed step's transforms at 128 x 256 x 1024: the axial ones (FP32 tiles of 16 lines,
     1024 threads x 16 elements covering nz = 1024) and the inverse theta one (tiles of 16 lines,
     256 threads x 8 elements covering ny / 2 = 128) take their persistent exact-tiling forms
-    (twiddles in LDS, buffer-addressed, the whole next tile loaded before the passes) where the
-    one-tile forms ran before (mask bit 2 leaves the FP32 divergence transform as it is); each
-    computes the same potential bit for bit"""
+    (twiddles in LDS, buffer-addressed, the whole next tile loaded before the passes), and the
+    divergence + theta transform its persistent form (twiddle tables in LDS, uz's lower faces
+    carried along a block's planes), where the one-tile forms ran before; each computes the same
+    potential bit for bit"""
     from simpletcsolver_amd import Solver
     nx, ny, nz = 128, 256, 1024
     small = make_case(nx=nx, ny=ny, nz=4, Re=100.0, amp=0.0, uy_walls=(0.0, 0.0))
