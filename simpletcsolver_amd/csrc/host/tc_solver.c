@@ -363,6 +363,7 @@ static void default_shapes(tc_solver_t * s){
   }
   s->predict_window = 1;
   s->fft_exact = 7;
+  s->psolve_lean = 1;
 }
 
 int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_solver_t ** out){
@@ -843,7 +844,7 @@ static int predict(tc_solver_t * s, int rkstep){
         xr[f] = L->yr; xv[f] = L->yv; xq[f] = L->yq1; xm[f] = L->ymeta;
       }
       PROF(TC_S_YSOLVE, tck_launch_psolve(&s->g, 1, nf, xd, xo, xc, xr, xv, xq, xm, s->tp, 1, xi0, xn,
-          s->mixed, s->stream));
+          s->mixed, s->psolve_lean, s->stream));
     }
     if(s->p.implicit[2]){
       for(int f = 0; f < nf; f++){
@@ -852,7 +853,7 @@ static int predict(tc_solver_t * s, int rkstep){
         xr[f] = L->zr; xv[f] = L->zv; xq[f] = L->zq1; xm[f] = L->zmeta;
       }
       PROF(TC_S_ZSOLVE, tck_launch_psolve(&s->g, 2, nf, xd, xo, xc, xr, xv, xq, xm, 1, 0, xi0, xn,
-          s->mixed, s->stream));
+          s->mixed, s->psolve_lean, s->stream));
     }
   }
   for(int f = 0; f < nf; f++) s->cur[f] = 1 - s->cur[f];
@@ -1137,7 +1138,8 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
  * block; results do not depend on them.  "predict_window" (kb 0 or 1) switches between forms
  * of the predictor, "fft_exact" (kb a mask: 1 the axial transforms, 2 the divergence + theta
  * transform's twiddles in LDS, 4 the inverse theta transform) between forms of the persistent
- * transforms; each pair computes the same bits */
+ * transforms, "psolve_lean" (kb 0 or 1) between the periodic line solve with row tests in every
+ * wave and the one whose interior waves skip them; each pair computes the same bits */
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
   if(kernel && strcmp(kernel, "predict_window") == 0){
     s->predict_window = kb ? 1 : 0;
@@ -1146,6 +1148,11 @@ int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
   }
   if(kernel && strcmp(kernel, "fft_exact") == 0){
     s->fft_exact = kb & 7;
+    s->gen++;
+    return 0;
+  }
+  if(kernel && strcmp(kernel, "psolve_lean") == 0){
+    s->psolve_lean = kb ? 1 : 0;
     s->gen++;
     return 0;
   }

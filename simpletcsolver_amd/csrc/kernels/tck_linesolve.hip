@@ -532,6 +532,8 @@ struct PSolveArgs {
   long long tstride;    /* table stride between rows (tp for y, 1 for z) */
   int tcol;             /* 1: tables indexed by i as well (y) */
   int dir;              /* 1: y, 2: z */
+  int nti, nsg;         /* block shape (TI lines, S segments), as launched */
+  int lean;             /* 1: waves of interior full-length segments take the lean form */
   int i0[TCK_XS_MAXF], isz[TCK_XS_MAXF];
 };
 
@@ -558,16 +560,27 @@ __device__ __forceinline__ void ps_rows(Q * out, const double * __restrict__ t, 
   }
 }
 
+/* plain store (as a halo image's) at byte offset vo + so of resource r */
+__device__ __forceinline__ void ps_bsti(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, double v){
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tck_u2v, v), r, vo, so, 0);
+}
+
 /* body with restrict-qualified pointers: loads of the old field and the tables may be
- * scheduled ahead of the stores to the increment array */
-template <typename D, int MAXM>
+ * scheduled ahead of the stores to the increment array.
+ * Two forms of the same operations, chosen per wave: the general one tests every row (row 0,
+ * row m-1, rows past the segment's count) and forms 64-bit addresses per row; the lean one,
+ * taken when every lane of the wave holds a full segment (MAXM rows) of interior rows, has no
+ * row tests and reaches the rows through buffer resources based at the wave's first lane, with
+ * a 32-bit offset per lane and a uniform one per row (the launcher checks that the offsets of
+ * a wave stay below 2^31).  The arithmetic of a row is the same in both. */
+template <typename D, int MAXM, bool LEAN>
 __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict__ d, double * uout,
     const double * __restrict__ ucur, const double * __restrict__ rt, const double * __restrict__ vt,
     const double * __restrict__ q1t, const double * __restrict__ meta, const int ib0, const int isz,
     double * ps_sm){
   const tc_geom_t & g = A.g;
-  const int TI = blockDim.x;
-  const int S = blockDim.y;
+  const int TI = A.nti;
+  const int S = A.nsg;
   const int ti = threadIdx.x;
   const int sg = threadIdx.y;
   /* tiles start at i = 1 (a 128 B line boundary); rows below i0 are skipped (ux: i0 = 2) */
@@ -581,7 +594,10 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   int cnt = m - row0;
   cnt = cnt < 0 ? 0 : (cnt > seg ? seg : cnt);
   const long long step = A.dir == 1 ? (long long)g.px : g.plane;
-  const int ic = valid ? i : ib0;
+  /* lanes outside the field's rows work on its nearest row (nothing of theirs is stored), so
+   * that the addresses of a wave do not decrease with the lane */
+  const int ihi = isz > 0 ? ib0 + isz - 1 : ib0;
+  const int ic = i < ib0 ? ib0 : (i > ihi ? ihi : i);
   const long long base = A.dir == 1 ? tck_idx(g, ic, 1, other) : tck_idx(g, ic, other, 1);
   const int col = A.tcol ? ic : 0;
   const long long ts = A.tstride;
@@ -599,26 +615,52 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   Q * smA = (Q *)ps_sm;                  /* [S][TI] */
   Q * smB = smA + S * TI;         /* [S][TI] */
   Q * smX = smA + 2 * S * TI;     /* [3][TI]: q0[0], q0[n-2], q[n-1] */
+  /* the lean form's addressing: element offsets from the wave's first lane (the smallest),
+   * one row further per `step` elements; table rows at (col + ts row) */
+  const long long lo = base + step * row0;
+  const long long w0 = tck_uni64(lo);
+  const unsigned vel = (unsigned)(lo - w0);
+  const unsigned sel = (unsigned)step;
+  unsigned vtab = (unsigned)((col + ts * row0) * 8);
+  const unsigned stab = (unsigned)(ts * 8);
   Q q[MAXM];
-  #pragma unroll
-  for(int e = 0; e < MAXM; e++) q[e] = (valid && e < cnt) ? Q(tck_ld(&d[base + step * (row0 + e)])) : Q(0.);
+  if constexpr(LEAN){
+    const __amdgpu_buffer_rsrc_t rd = tck_rsrc_u(d + w0, 0xFFFFFFFFu);
+    #pragma unroll
+    for(int e = 0; e < MAXM; e++)
+      q[e] = Q(tck_bldn<D>(rd, vel * (unsigned)sizeof(D), (unsigned)e * sel * (unsigned)sizeof(D)));
+  }else{
+    #pragma unroll
+    for(int e = 0; e < MAXM; e++) q[e] = (valid && e < cnt) ? Q(tck_ld(&d[base + step * (row0 + e)])) : Q(0.);
+  }
   if(valid && sg == S - 1) smX[2 * TI + ti] = Q(d[base + step * m]);
   Q t[PS_PRE];
   /* forward: q0 = q0 / c0, q_i = r_i (q_i - l q_{i-1}) */
   Q a = Q(1.), b = Q(0.);
-  #pragma unroll
-  for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
-    if(e0 < cnt){
-      ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
-      #pragma unroll
-      for(int e = 0; e < PS_PRE; e++){
-        if(e0 + e < cnt){
-          const int row = row0 + e0 + e;
-          if(row == 0){ a = Q(0.); b = q[e0 + e] / t[e]; }
-          else {
-            const Q ai = - t[e] * tl;
-            b = ai * b + t[e] * q[e0 + e];
-            a = ai * a;
+  if constexpr(LEAN){
+    /* rows from jc on: the factors are the constants */
+    #pragma unroll
+    for(int e = 0; e < MAXM; e++){
+      asm volatile("" : "+v"(q[e]));   /* row by row: keeps the products from being formed all ahead */
+      const Q ai = - rc * tl;
+      b = ai * b + rc * q[e];
+      a = ai * a;
+    }
+  }else{
+    #pragma unroll
+    for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
+      if(e0 < cnt){
+        ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          if(e0 + e < cnt){
+            const int row = row0 + e0 + e;
+            if(row == 0){ a = Q(0.); b = q[e0 + e] / t[e]; }
+            else {
+              const Q ai = - t[e] * tl;
+              b = ai * b + t[e] * q[e0 + e];
+              a = ai * a;
+            }
           }
         }
       }
@@ -629,18 +671,27 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   __syncthreads();
   Q in = Q(0.);
   for(int s2 = 0; s2 < sg; s2++) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
-  #pragma unroll
-  for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
-    if(e0 < cnt){
-      ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
-      #pragma unroll
-      for(int e = 0; e < PS_PRE; e++){
-        if(e0 + e < cnt){
-          const int row = row0 + e0 + e;
-          if(row == 0) q[e0 + e] = q[e0 + e] / t[e];
-          else q[e0 + e] = t[e] * (q[e0 + e] - tl * in);
-          in = q[e0 + e];
-          if(row == m - 1) smX[TI + ti] = q[e0 + e];
+  if constexpr(LEAN){
+    #pragma unroll
+    for(int e = 0; e < MAXM; e++){
+      asm volatile("" : "+v"(q[e]));
+      q[e] = rc * (q[e] - tl * in);
+      in = q[e];
+    }
+  }else{
+    #pragma unroll
+    for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
+      if(e0 < cnt){
+        ps_rows(t, r, ts, row0, e0, jc, m, rc, rlast);
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          if(e0 + e < cnt){
+            const int row = row0 + e0 + e;
+            if(row == 0) q[e0 + e] = q[e0 + e] / t[e];
+            else q[e0 + e] = t[e] * (q[e0 + e] - tl * in);
+            in = q[e0 + e];
+            if(row == m - 1) smX[TI + ti] = q[e0 + e];
+          }
         }
       }
     }
@@ -649,19 +700,29 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   /* backward: q_i -= v_i q_{i+1}, q0[n-2] kept */
   a = Q(1.);
   b = Q(0.);
-  #pragma unroll
-  for(int e0 = MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
-    if(e0 < cnt){
-      ps_rows(t, v, ts, row0, e0, jc, m, vc, Q(0.));
-      #pragma unroll
-      for(int e = PS_PRE - 1; e >= 0; e--){
-        if(e0 + e < cnt){
-          const int row = row0 + e0 + e;
-          if(row == m - 1){ a = Q(0.); b = q[e0 + e]; }
-          else {
-            const Q ai = - t[e];
-            b = ai * b + q[e0 + e];
-            a = ai * a;
+  if constexpr(LEAN){
+    #pragma unroll
+    for(int e = MAXM - 1; e >= 0; e--){
+      asm volatile("" : "+v"(q[e]));
+      const Q ai = - vc;
+      b = ai * b + q[e];
+      a = ai * a;
+    }
+  }else{
+    #pragma unroll
+    for(int e0 = MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
+      if(e0 < cnt){
+        ps_rows(t, v, ts, row0, e0, jc, m, vc, Q(0.));
+        #pragma unroll
+        for(int e = PS_PRE - 1; e >= 0; e--){
+          if(e0 + e < cnt){
+            const int row = row0 + e0 + e;
+            if(row == m - 1){ a = Q(0.); b = q[e0 + e]; }
+            else {
+              const Q ai = - t[e];
+              b = ai * b + q[e0 + e];
+              a = ai * a;
+            }
           }
         }
       }
@@ -672,16 +733,25 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   __syncthreads();
   in = Q(0.);
   for(int s2 = S - 1; s2 > sg; s2--) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
-  #pragma unroll
-  for(int e0 = MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
-    if(e0 < cnt){
-      ps_rows(t, v, ts, row0, e0, jc, m, vc, Q(0.));
-      #pragma unroll
-      for(int e = PS_PRE - 1; e >= 0; e--){
-        if(e0 + e < cnt){
-          const int row = row0 + e0 + e;
-          if(row != m - 1) q[e0 + e] -= t[e] * in;
-          in = q[e0 + e];
+  if constexpr(LEAN){
+    #pragma unroll
+    for(int e = MAXM - 1; e >= 0; e--){
+      asm volatile("" : "+v"(q[e]));
+      q[e] -= vc * in;
+      in = q[e];
+    }
+  }else{
+    #pragma unroll
+    for(int e0 = MAXM - PS_PRE; e0 >= 0; e0 -= PS_PRE){
+      if(e0 < cnt){
+        ps_rows(t, v, ts, row0, e0, jc, m, vc, Q(0.));
+        #pragma unroll
+        for(int e = PS_PRE - 1; e >= 0; e--){
+          if(e0 + e < cnt){
+            const int row = row0 + e0 + e;
+            if(row != m - 1) q[e0 + e] -= t[e] * in;
+            in = q[e0 + e];
+          }
         }
       }
     }
@@ -692,6 +762,86 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   const Q num = smX[2 * TI + ti] - tu * smX[ti] - tl * smX[TI + ti];
   const Q xl = fabs(double(den)) < DBL_EPSILON ? Q(0.) : num / den;
   if(!valid) return;
+  if constexpr(LEAN){
+    /* interior rows only: no row m, and the halo images are those of the other index (a
+     * uniform offset, when it is at a wall of the period) */
+    const __amdgpu_buffer_rsrc_t rq = tck_rsrc_u(q1t, 0xFFFFFFFFu);
+    unsigned vd = vel * (unsigned)sizeof(D);
+    const unsigned sd = sel * (unsigned)sizeof(D);
+    unsigned vu = vel * 8u;
+    const unsigned su = sel * 8u;
+    long long dimg = 0;
+    if(A.dir == 1){
+      if(other == 1) dimg = (long long)g.nz * g.plane;
+      else if(other == g.nz) dimg = -(long long)g.nz * g.plane;
+    }else{
+      if(other == 1) dimg = (long long)g.ny * g.px;
+      else if(other == g.ny) dimg = -(long long)g.ny * g.px;
+    }
+    const __amdgpu_buffer_rsrc_t rd = tck_rsrc_u(d + w0, 0xFFFFFFFFu);
+    const __amdgpu_buffer_rsrc_t rc2 = tck_rsrc_u(ucur ? ucur + w0 : q1t, 0xFFFFFFFFu);
+    const __amdgpu_buffer_rsrc_t ro = tck_rsrc_u(uout ? uout + w0 : q1t, 0xFFFFFFFFu);
+    const __amdgpu_buffer_rsrc_t ri = tck_rsrc_u(uout ? uout + w0 + dimg : q1t, 0xFFFFFFFFu);
+    if constexpr(sizeof(Q) == 8){
+      /* every load before the first store, the results formed in place in q */
+      #pragma unroll
+      for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
+        Q qq[PS_PRE];
+        double uu[PS_PRE];
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          qq[e] = Q(tck_bld<double>(rq, vtab, (unsigned)(e0 + e) * stab));
+          uu[e] = ucur ? tck_bldn<double>(rc2, vu, (unsigned)(e0 + e) * su) : 0.;
+        }
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          const Q x = q[e0 + e] + xl * qq[e];
+          q[e0 + e] = ucur ? Q(uu[e] + double(x)) : x;
+        }
+        /* one group's loads in registers at a time (all of them beside q would not fit): the
+         * next group's offsets only after this group's results */
+        static_assert(PS_PRE == 8, "row group of 8");
+        asm volatile("" : "+v"(vtab), "+v"(vu), "+v"(q[e0]), "+v"(q[e0 + 1]), "+v"(q[e0 + 2]), "+v"(q[e0 + 3]),
+            "+v"(q[e0 + 4]), "+v"(q[e0 + 5]), "+v"(q[e0 + 6]), "+v"(q[e0 + 7]));
+      }
+      if(ucur){
+        #pragma unroll
+        for(int e = 0; e < MAXM; e++) tck_bstn<double>(ro, vu, (unsigned)e * su, double(q[e]));
+        if(dimg != 0){
+          #pragma unroll
+          for(int e = 0; e < MAXM; e++) ps_bsti(ri, vu, (unsigned)e * su, double(q[e]));
+        }
+      }else{
+        #pragma unroll
+        for(int e = 0; e < MAXM; e++) tck_bstn<D>(rd, vd, (unsigned)e * sd, D(q[e]));
+      }
+    }else{
+      /* q1 and the old field, PS_PRE rows at a time */
+      #pragma unroll
+      for(int e0 = 0; e0 < MAXM; e0 += PS_PRE){
+        Q qq[PS_PRE];
+        double uu[PS_PRE];
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          qq[e] = Q(tck_bld<double>(rq, vtab, (unsigned)(e0 + e) * stab));
+          uu[e] = ucur ? tck_bldn<double>(rc2, vu, (unsigned)(e0 + e) * su) : 0.;
+        }
+        #pragma unroll
+        for(int e = 0; e < PS_PRE; e++){
+          const Q x = q[e0 + e] + xl * qq[e];
+          if(ucur){
+            const double y = uu[e] + double(x);
+            tck_bstn<double>(ro, vu, (unsigned)(e0 + e) * su, y);
+            if(dimg != 0) ps_bsti(ri, vu, (unsigned)(e0 + e) * su, y);
+          }else{
+            tck_bstn<D>(rd, vd, (unsigned)(e0 + e) * sd, D(x));
+          }
+        }
+        asm volatile("" : "+v"(vtab), "+v"(vu), "+v"(vd));   /* the next group's loads after these stores */
+      }
+    }
+    return;
+  }
   if constexpr(sizeof(Q) == 8){
     /* FP64: every load of the pass (q1, the old field) before the first store, the results
      * formed in place in q; then the stores.  A load issued after a store could be waited for
@@ -780,11 +930,26 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
 }
 
 template <typename D, int MAXM>
-__global__ void __launch_bounds__(512) tck_psolve(const PSolveArgs A){
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) tck_psolve(const PSolveArgs A){
   extern __shared__ double ps_sm[];
   const int f = blockIdx.z;
-  psolve_body<D, MAXM>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.q1[f], A.meta[f],
-      A.i0[f], A.isz[f], ps_sm);
+  /* the lean form where every lane of the wave holds a full segment of interior rows at or
+   * past the convergence row jc (constant factors; a uniform branch: each form passes the
+   * same three barriers) */
+  const int m = (A.dir == 1 ? A.g.ny : A.g.nz) - 1;
+  const int seg = (m + A.nsg - 1) / A.nsg;
+  const int row0 = (int)threadIdx.y * seg;
+  const int i = 1 + blockIdx.x * A.nti + threadIdx.x;
+  const int ib0 = A.i0[f], ihi = A.isz[f] > 0 ? ib0 + A.isz[f] - 1 : ib0;
+  const int ic = i < ib0 ? ib0 : (i > ihi ? ihi : i);
+  const int jc = (int)A.meta[f][3 * A.tstride + (A.tcol ? ic : 0)];
+  const bool full = A.lean && seg == MAXM && row0 > 0 && row0 >= jc && row0 + MAXM <= m - 1;
+  if(__builtin_amdgcn_ballot_w64(!full) == 0)
+    psolve_body<D, MAXM, true>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.q1[f], A.meta[f],
+        A.i0[f], A.isz[f], ps_sm);
+  else
+    psolve_body<D, MAXM, false>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.q1[f], A.meta[f],
+        A.i0[f], A.isz[f], ps_sm);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -1009,7 +1174,7 @@ int tck_launch_xsolve(const tc_geom_t * g, int nf, void * const * d, double * co
 int tck_launch_psolve(const tc_geom_t * g, int dir, int nf, void * const * d, double * const * uout,
     const double * const * ucur, const double * const * r, const double * const * v,
     const double * const * q1, const double * const * meta, long long tstride, int tcol,
-    const int * i0, const int * isz, int mixed, hipStream_t stream){
+    const int * i0, const int * isz, int mixed, int lean, hipStream_t stream){
   if(nf < 1 || nf > TCK_XS_MAXF) return (int)hipErrorInvalidValue;
   PSolveArgs A;
   A.g = *g;
@@ -1035,6 +1200,15 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, int nf, void * const * d, do
   const int nother = dir == 1 ? g->nz : g->ny;
   const size_t lds = sizeof(double) * (size_t)(2 * S * TI + 3 * TI);
   const dim3 grd((span + TI - 1) / TI, nother, nf);
+  A.nti = TI;
+  A.nsg = S;
+  /* the lean form's byte offsets: within a wave (its lanes span at most 64 / TI + 2 segments
+   * and TI columns) plus a segment's rows, and the table rows, all below 2^31 */
+  const long long step = dir == 1 ? (long long)g->px : g->plane;
+  const long long seg = (m + S - 1) / S;
+  const long long wspan = ((long long)(64 / TI + 2) * seg + maxm) * step + 64;
+  const long long tspan = ((long long)(tcol ? tstride : 1) * (m + 1) + g->px) * 8;
+  A.lean = lean && wspan * 8 < (1LL << 31) && tspan < (1LL << 31);
   if(mixed) hipLaunchKernelGGL((tck_psolve<float, 2 * PS_MAXM>), grd, dim3(TI, S), lds, stream, A);
   else hipLaunchKernelGGL((tck_psolve<double, PS_MAXM>), grd, dim3(TI, S), lds, stream, A);
   TCK_CHECK_LAUNCH();

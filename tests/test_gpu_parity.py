@@ -61,6 +61,36 @@ def test_predict_substep_matches_oracle(need_gpu, implicit, window):
         _cmp(o, s, VEL + ("t",), case, tol)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lean_line_solves_match(need_gpu, precision):
+    """At 32 x 256 x 256 the periodic y and z solves split each line into full segments (32
+    rows in FP64, 64 in the mixed step's FP32), and the waves whose segments are all interior
+    and past the factors' convergence row take the lean form (no row tests, buffer-addressed).
+    Both forms give the same bits over the three substeps.  In FP64 they also match the oracle
+    to 1e-14."""
+    case = make_case(nx=32, ny=256, nz=256, implicit=(1, 1, 1), t_amp=1e-2, amp=0.1)
+    from simpletcsolver_amd import Solver
+    out = []
+    for lean in (0, 1, 1):
+        s = Solver(case["nx"], case["ny"], case["nz"], case["xf"], case["xc"], precision=precision,
+                   **case["kw"])
+        s.load_fields(case["fields"])
+        s.tune("psolve_lean", lean, 0)
+        for rk in range(3):
+            s.predict(rk, 1e-3)
+        out.append(s.fields())
+        if precision == "fp64" and len(out) == 2:
+            o = oracle_for(case)
+            for rk in range(3):
+                o.predict(rk, 1e-3)
+            _cmp(o, s, VEL + ("t",), case, 1e-14)
+        s.close()
+    for n in VEL + ("t",):
+        assert np.abs(out[0][n]).max() > 0.0
+        for k in (1, 2):
+            assert np.array_equal(out[k][n], out[0][n]), (n, k)
+
+
 @pytest.mark.parametrize("nx,ny,nz", [(70, 24, 12), (33, 8, 6), (40, 36, 12)])
 def test_window_predictor_tiles_match_oracle(need_gpu, nx, ny, nz):
     """the LDS-window predictor over several 32 x 8 tiles, partial in i and j: bit for bit"""
