@@ -119,9 +119,10 @@ int tc_stat_get(tc_solver_t * s, const char * name, double * host, size_t * num)
  * kb a mask (1: the axial transforms, 4: the inverse theta transform: their exact-tiling
  * forms where the tiles cover the spectrum exactly; 2: the divergence + theta transform's
  * twiddle tables in LDS; 7, the default: all three) the forms of the persistent FP64
- * transforms; "psolve_lean" with kb 1 (the default) or 0 lets the periodic y and z solves'
- * waves of interior full-length segments past the factors' convergence row skip the row
- * tests, or not.  Results are identical for every shape and form; only the speed changes */
+ * transforms; "psolve_lean" with kb a mask (1: the periodic y and z solves' waves of interior
+ * full-length segments past the factors' convergence row skip the row tests; 2: their folds
+ * read LDS four segments at a time; 3, the default: both) the forms of the periodic solves.
+ * Results are identical for every shape and form; only the speed changes */
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty);
 /* no reference counterpart: times the predictor for every candidate shape (kb in 2..32,
  * ty in 1, 2, 4) on the solver's own buffers, reps launches each, and keeps the fastest for

@@ -91,7 +91,7 @@ struct tc_solver_s {
   int tune[TC_NTUNE][2];
   int predict_window;     /* 1: the predictor variant with the fields' planes staged in LDS */
   int fft_exact;          /* forms of the persistent transforms where they apply: 1 axial exact, 2 divergence twiddles in LDS, 4 inverse theta exact */
-  int psolve_lean;        /* 1: the periodic line solves' interior waves skip the row tests */
+  int psolve_lean;        /* periodic line solves: 1 interior waves skip the row tests, 2 batched LDS folds */
   /* bumped by every host-side change that work recorded earlier (an ensemble's graphs)
    * cannot see: fields replaced, a closure coupled, uncoupled or destroyed, launch shapes,
    * dt parameters, diagnostics or profiling armed, and every launch outside a replay */

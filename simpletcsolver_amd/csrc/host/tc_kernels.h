@@ -42,8 +42,9 @@ int tck_launch_xsolve(const tc_geom_t * g, int nf, void * const * d, double * co
     const double * const * ucur, const double * const * r, const double * const * v,
     const double * const * l, const int * i0, const int * n, int mixed, hipStream_t stream);
 /* the periodic solves (dir 1: y, 2: z) of nf (1..4) fields in one launch, with per-field
- * arrays as tck_launch_xsolve's plus the correction vector q1 and the meta rows; lean 1 lets
- * waves of interior full-length segments skip the row tests (same results) */
+ * arrays as tck_launch_xsolve's plus the correction vector q1 and the meta rows; lean bit 1
+ * lets waves of interior full-length segments skip the row tests, bit 2 batches the folds' LDS
+ * reads (same results) */
 int tck_launch_psolve(const tc_geom_t * g, int dir, int nf, void * const * d, double * const * uout,
     const double * const * ucur, const double * const * r, const double * const * v,
     const double * const * q1, const double * const * meta, long long tstride, int tcol,

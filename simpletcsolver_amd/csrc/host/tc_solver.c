@@ -363,7 +363,7 @@ static void default_shapes(tc_solver_t * s){
   }
   s->predict_window = 1;
   s->fft_exact = 7;
-  s->psolve_lean = 1;
+  s->psolve_lean = 3;
 }
 
 int tc_create(const tc_params_t * p, const double * xf, const double * xc, tc_solver_t ** out){
@@ -1138,8 +1138,9 @@ int tc_dissipation(tc_solver_t * s, double out[2]){
  * block; results do not depend on them.  "predict_window" (kb 0 or 1) switches between forms
  * of the predictor, "fft_exact" (kb a mask: 1 the axial transforms, 2 the divergence + theta
  * transform's twiddles in LDS, 4 the inverse theta transform) between forms of the persistent
- * transforms, "psolve_lean" (kb 0 or 1) between the periodic line solve with row tests in every
- * wave and the one whose interior waves skip them; each pair computes the same bits */
+ * transforms, "psolve_lean" (kb a mask: 1 the periodic line solves' interior waves skip the row
+ * tests, 2 their folds read LDS four segments at a time; default 3) between forms of the
+ * periodic line solves; each pair computes the same bits */
 int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
   if(kernel && strcmp(kernel, "predict_window") == 0){
     s->predict_window = kb ? 1 : 0;
@@ -1152,7 +1153,7 @@ int tc_tune_set(tc_solver_t * s, const char * kernel, int kb, int ty){
     return 0;
   }
   if(kernel && strcmp(kernel, "psolve_lean") == 0){
-    s->psolve_lean = kb ? 1 : 0;
+    s->psolve_lean = kb & 3;
     s->gen++;
     return 0;
   }

@@ -533,7 +533,8 @@ struct PSolveArgs {
   int tcol;             /* 1: tables indexed by i as well (y) */
   int dir;              /* 1: y, 2: z */
   int nti, nsg;         /* block shape (TI lines, S segments), as launched */
-  int lean;             /* 1: waves of interior full-length segments take the lean form */
+  int lean;             /* bit 1: waves of interior full-length segments take the lean form;
+                           bit 2: the folds read LDS four segments at a time */
   int i0[TCK_XS_MAXF], isz[TCK_XS_MAXF];
 };
 
@@ -669,8 +670,20 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   smA[sg * TI + ti] = a;
   smB[sg * TI + ti] = b;
   __syncthreads();
+  /* the fold over the earlier segments, in order; LDS reads four segments at a time so that
+   * one wait covers them */
   Q in = Q(0.);
-  for(int s2 = 0; s2 < sg; s2++) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
+  {
+    int s2 = 0;
+    if(A.lean & 2) for(; s2 + 4 <= sg; s2 += 4){
+      Q fa[4], fb[4];
+      #pragma unroll
+      for(int u = 0; u < 4; u++){ fa[u] = smA[(s2 + u) * TI + ti]; fb[u] = smB[(s2 + u) * TI + ti]; }
+      #pragma unroll
+      for(int u = 0; u < 4; u++) in = fa[u] * in + fb[u];
+    }
+    for(; s2 < sg; s2++) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
+  }
   if constexpr(LEAN){
     #pragma unroll
     for(int e = 0; e < MAXM; e++){
@@ -732,7 +745,17 @@ __device__ __forceinline__ void psolve_body(const PSolveArgs & A, D * __restrict
   smB[sg * TI + ti] = b;
   __syncthreads();
   in = Q(0.);
-  for(int s2 = S - 1; s2 > sg; s2--) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
+  {
+    int s2 = S - 1;
+    if(A.lean & 2) for(; s2 - 4 >= sg; s2 -= 4){
+      Q fa[4], fb[4];
+      #pragma unroll
+      for(int u = 0; u < 4; u++){ fa[u] = smA[(s2 - u) * TI + ti]; fb[u] = smB[(s2 - u) * TI + ti]; }
+      #pragma unroll
+      for(int u = 0; u < 4; u++) in = fa[u] * in + fb[u];
+    }
+    for(; s2 > sg; s2--) in = smA[s2 * TI + ti] * in + smB[s2 * TI + ti];
+  }
   if constexpr(LEAN){
     #pragma unroll
     for(int e = MAXM - 1; e >= 0; e--){
@@ -943,7 +966,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) t
   const int ib0 = A.i0[f], ihi = A.isz[f] > 0 ? ib0 + A.isz[f] - 1 : ib0;
   const int ic = i < ib0 ? ib0 : (i > ihi ? ihi : i);
   const int jc = (int)A.meta[f][3 * A.tstride + (A.tcol ? ic : 0)];
-  const bool full = A.lean && seg == MAXM && row0 > 0 && row0 >= jc && row0 + MAXM <= m - 1;
+  const bool full = (A.lean & 1) && seg == MAXM && row0 > 0 && row0 >= jc && row0 + MAXM <= m - 1;
   if(__builtin_amdgcn_ballot_w64(!full) == 0)
     psolve_body<D, MAXM, true>(A, (D *)A.d[f], A.uout[f], A.ucur[f], A.r[f], A.v[f], A.q1[f], A.meta[f],
         A.i0[f], A.isz[f], ps_sm);
@@ -1208,7 +1231,7 @@ int tck_launch_psolve(const tc_geom_t * g, int dir, int nf, void * const * d, do
   const long long seg = (m + S - 1) / S;
   const long long wspan = ((long long)(64 / TI + 2) * seg + maxm) * step + 64;
   const long long tspan = ((long long)(tcol ? tstride : 1) * (m + 1) + g->px) * 8;
-  A.lean = lean && wspan * 8 < (1LL << 31) && tspan < (1LL << 31);
+  A.lean = (lean & 2) | ((lean & 1) && wspan * 8 < (1LL << 31) && tspan < (1LL << 31) ? 1 : 0);
   if(mixed) hipLaunchKernelGGL((tck_psolve<float, 2 * PS_MAXM>), grd, dim3(TI, S), lds, stream, A);
   else hipLaunchKernelGGL((tck_psolve<double, PS_MAXM>), grd, dim3(TI, S), lds, stream, A);
   TCK_CHECK_LAUNCH();
